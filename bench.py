@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Benchmark: emulated Bine allreduce on one MI355X.
+
+Workload: P = 512 virtual ranks on an 8x8x8 torus, fp64 SUM, 524288 elements
+(4 MiB) per rank, inputs N(0,1) resident in HBM.  One step is one complete
+allreduce of all 512 ranks.  The measured path is the batched execution: the
+whole Bine schedule compiled into one fused gather-reduce launch (replaying
+the reference's reduction tree for every element), captured in a HIP graph,
+launch shape picked by the autotuner.  The plain per-step path (one kernel per
+schedule step, 1 + 9 + 9 launches) is timed beside it.
+
+Algorithmic bytes per allreduce: 2 * P * S = 4 GiB (read every rank's input
+once, write every rank's output once).  GB/s = algorithmic bytes / time.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "emulated Bine allreduce GB/s & %HBM peak, P=512 x 4 MiB; batched vs per-step"
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, 8 TB/s
+DIMS = [8, 8, 8]
+P = 512
+COUNT = 524288  # fp64 elements per rank = 4 MiB
+ELEM = 8
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
+    return ap.parse_args()
+
+
+def time_launches(fn, stream, steps, warmup):
+    """Wall time per step (sync-bracketed) and HIP-event time on the launch stream."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    return wall * 1e3, e0.elapsed_time(e1) / steps
+
+
+def cpu_baseline(budget_s):
+    """The oracle (single-threaded C restatement of the reference's MPI-layer
+    allreduce B over P virtual ranks) on a bounded sample of the workload."""
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    count = 512 * 128  # 512 KiB per rank: same torus and block structure, data movement dominates
+    x = np.random.default_rng(0).standard_normal((P, count))
+    O.allreduce(x[:, :P], DIMS)  # load + warm
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        O.allreduce(x, DIMS)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or reps >= 50:
+            break
+    per = el / reps
+    gbps = 2.0 * P * count * ELEM / per / 1e9
+    return {"value": round(gbps, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"oracle allreduce B, P=512 on 8x8x8, fp64, {count} elements (512 KiB) per rank, "
+                      f"{reps} runs, {per:.3f} s each; GB/s = 2*P*S / time"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the P virtual ranks live in one MI355X's HBM; "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        return 2
+    import torch
+    from bine_trees_fugaku_amd import Batch, VComm
+
+    torch.cuda.set_device(0)
+    stream = torch.cuda.Stream()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1234)
+    send = torch.randn((P, COUNT), dtype=torch.float64, device="cuda", generator=g)
+    recv = torch.empty_like(send)
+    recv_step = torch.empty_like(send)
+    b_alg = 2 * P * COUNT * ELEM
+
+    with torch.cuda.stream(stream):
+        fused = VComm(DIMS, exec="FUSED")
+        batch = Batch(fused, [{"coll": "allreduce", "sendbuf": send, "recvbuf": recv, "count": COUNT}])
+        batch.autotune(iters=3, stream=stream)
+        tuning = batch.tuning()
+        bstats = batch.stats()
+        wall_ms, ev_ms = time_launches(lambda: batch.launch_graph(stream), stream, args.steps, args.warmup)
+
+        step = VComm(DIMS, exec="STEP")
+        step.autotune("allreduce", send, recv_step, COUNT, exec="STEP", iters=2, stream=stream)
+        sstats = step.plan_stats("allreduce", COUNT, "float64", exec="STEP")
+        s_wall, s_ev = time_launches(lambda: step.allreduce(send, recv_step, stream=stream), stream,
+                                     max(3, args.steps // 4), 2)
+    torch.cuda.synchronize()
+    same = bool(torch.equal(recv.view(torch.int64), recv_step.view(torch.int64)))
+
+    gbps = b_alg / (wall_ms * 1e-3) / 1e9
+    ach = b_alg / (ev_ms * 1e-3) / 1e9
+    step_gbps = b_alg / (s_wall * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_allreduce_8x8x8_fp64.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    line = {
+        "metric": METRIC, "value": round(gbps, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall_ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic N(0,1), seeded",
+        "config": {
+            "workload": "allreduce_8x8x8_fp64_4MiB_per_rank", "P": P, "torus": "8x8x8", "elements_per_rank": COUNT,
+            "bytes_per_rank": COUNT * ELEM, "algorithmic_bytes": b_alg, "exec": "batched: fused + hipGraph",
+            "tuning": tuning, "batched_launches": bstats["launches"], "batched_items": bstats["items"],
+            "per_step_GBps": round(step_gbps, 2), "per_step_ms": round(s_wall, 4),
+            "per_step_launches": sstats["launches"], "per_step_schedule_bytes": sstats["bytes_read"] + sstats["bytes_written"],
+            "batched_vs_per_step": round(s_wall / wall_ms, 3), "pct_hbm_peak": round(100 * gbps / HBM_PEAK_GBPS, 2),
+            "batched_equals_per_step_bitwise": same,
+        },
+        "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": traffic},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+    batch.close()
+    fused.close()
+    step.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,970 @@
+// api.cpp -- C ABI of libbine_hip.so (declared in include/bine_hip.h).
+//
+// A vcomm owns the torus, the per-collective configuration and a cache of
+// compiled schedules.  A schedule is compiled once per (collective, count,
+// type, op, root): the step program is built, checked symbolically, lowered
+// to work items and uploaded with its temporary buffers.  A call then only
+// validates its buffers and enqueues kernel launches on the caller's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "bine_hip.h"
+#include "gred.hpp"
+#include "plan.hpp"
+
+namespace bine {
+hipError_t launch_gred(int dtype, int max_lg, const GredItem* items, uint32_t n, const PoolEntry* pool,
+                       const BufTable& tab, const BufTable* tabs, bool batch, uint32_t allow_vec,
+                       const LaunchShape& sh, hipStream_t st);
+hipError_t launch_reduce_local(int dtype, const void* in, void* inout, uint64_t n, hipStream_t st);
+}  // namespace bine
+
+using namespace bine;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                         \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess) return fail(BINE_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+size_t dtsize(int dt) {
+    switch (dt) {
+    case BINE_INT32: return 4;
+    case BINE_INT64: return 8;
+    case BINE_FLOAT: return 4;
+    case BINE_DOUBLE: return 8;
+    case BINE_INT8: return 1;
+    default: return 0;
+    }
+}
+
+struct DevArrays {
+    Lowered low;
+    GredItem* items = nullptr;
+    PoolEntry* pool = nullptr;
+    int max_lg = 0;
+    int64_t tile_elems = 0;
+    LaunchShape shape;
+    void release() {
+        if (items) (void)hipFree(items);
+        if (pool) (void)hipFree(pool);
+        items = nullptr;
+        pool = nullptr;
+        low = Lowered();
+    }
+};
+
+int max_lg_of(const std::vector<std::vector<Group>>& gs) {
+    int m = 0;
+    for (const auto& v : gs)
+        for (const auto& g : v) m = std::max<int>(m, g.lg);
+    return m;
+}
+
+int upload(const std::vector<std::vector<Group>>& gs, size_t ds, int64_t tile_elems, uint32_t bucket, DevArrays* d) {
+    d->release();
+    lower_groups(gs, tile_elems, (int)(16 / ds), bucket, ds, &d->low);
+    d->max_lg = max_lg_of(gs);
+    d->tile_elems = tile_elems;
+    if (!d->low.items.empty()) {
+        HIPCHK(hipMalloc(&d->items, d->low.items.size() * sizeof(GredItem)));
+        HIPCHK(hipMemcpy(d->items, d->low.items.data(), d->low.items.size() * sizeof(GredItem), hipMemcpyHostToDevice));
+    }
+    if (!d->low.pool.empty()) {
+        HIPCHK(hipMalloc(&d->pool, d->low.pool.size() * sizeof(PoolEntry)));
+        HIPCHK(hipMemcpy(d->pool, d->low.pool.data(), d->low.pool.size() * sizeof(PoolEntry), hipMemcpyHostToDevice));
+    }
+    return BINE_OK;
+}
+
+struct DevPlan {
+    CompiledPlan cp;
+    int dtype = 0;
+    size_t ds = 0;
+    DevArrays step, fused;
+    void* tmp[2] = {nullptr, nullptr};  // BUF_TMP, BUF_TMP2 workspaces
+    ~DevPlan() {
+        step.release();
+        fused.release();
+        for (void* p : tmp)
+            if (p) (void)hipFree(p);
+    }
+};
+
+struct PlanKey {
+    int coll, dtype, op, root;
+    int64_t count;
+    bool operator==(const PlanKey& o) const {
+        return coll == o.coll && dtype == o.dtype && op == o.op && root == o.root && count == o.count;
+    }
+};
+struct PlanKeyHash {
+    size_t operator()(const PlanKey& k) const {
+        uint64_t h = (uint64_t)k.count * 0x9e3779b97f4a7c15ull;
+        h ^= ((uint64_t)k.coll << 48) ^ ((uint64_t)k.dtype << 40) ^ ((uint64_t)k.op << 32) ^ (uint32_t)k.root;
+        return (size_t)(h ^ (h >> 29));
+    }
+};
+
+int64_t default_tile_elems(size_t ds) { return (int64_t)(4096 / ds); }
+
+}  // namespace
+
+struct bine_vcomm_s {
+    bine_vcomm_config_t cfg;
+    Torus t;
+    std::mutex mu;
+    std::unordered_map<PlanKey, std::unique_ptr<DevPlan>, PlanKeyHash> plans;
+};
+
+namespace {
+
+// Compile (or fetch) the schedule for a request.
+int get_plan(bine_vcomm_t comm, const CollRequest& req, int dtype, DevPlan** out) {
+    const size_t ds = dtsize(dtype);
+    if (!ds) return fail(BINE_ERR_ARG, "unknown datatype");
+    PlanKey key{req.coll, dtype, req.op, req.root, req.count};
+    std::lock_guard<std::mutex> lock(comm->mu);
+    auto it = comm->plans.find(key);
+    if (it != comm->plans.end()) {
+        *out = it->second.get();
+        return BINE_OK;
+    }
+    HIPCHK(hipSetDevice(comm->cfg.device));
+    Program prog;
+    std::string err;
+    if (!build_program(comm->t, comm->cfg, req, &prog, &err)) return fail(BINE_ERR_UNSUPPORTED, err);
+    for (int b = 0; b < kNumBufs; ++b)
+        if ((uint64_t)prog.extent[b] >= (1ull << 32)) return fail(BINE_ERR_UNSUPPORTED, "per-rank buffer exceeds 2^32 elements");
+    auto dp = std::make_unique<DevPlan>();
+    if (!compile_program(prog, true, &dp->cp, &err)) return fail(BINE_ERR_UNSUPPORTED, err);
+    dp->dtype = dtype;
+    dp->ds = ds;
+    const int64_t tile = default_tile_elems(ds);
+    int rc = upload(dp->cp.step_groups, ds, tile, 0, &dp->step);
+    if (rc) return rc;
+    if (dp->cp.fused_ok && max_lg_of(std::vector<std::vector<Group>>(1, dp->cp.fused)) > 14) {
+        dp->cp.fused_ok = false;  // the kernel's register stack holds trees of up to 2^14 leaves
+        dp->cp.fused.clear();
+        dp->cp.fused_why = "reduction tree wider than 2^14 leaves";
+    }
+    if (dp->cp.fused_ok) {
+        std::vector<std::vector<Group>> one(1, dp->cp.fused);
+        rc = upload(one, ds, tile, 0, &dp->fused);
+        if (rc) return rc;
+    }
+    // temporaries are allocated only when some step touches them
+    bool used[kNumBufs] = {false, false, false, false};
+    for (const auto& launch : dp->cp.step_groups)
+        for (const Group& g : launch) {
+            for (const PoolEntry& e : g.leaves) used[e.rb >> 28] = true;
+            for (const PoolEntry& e : g.outs) used[e.rb >> 28] = true;
+        }
+    for (int b = BUF_TMP; b <= BUF_TMP2; ++b) {
+        if (prog.extent[b] > 0 && used[b]) {
+            size_t bytes = (size_t)prog.extent[b] * ds * (prog.single[b] ? 1 : prog.P);
+            HIPCHK(hipMalloc(&dp->tmp[b - BUF_TMP], bytes));
+        }
+    }
+    *out = dp.get();
+    comm->plans.emplace(key, std::move(dp));
+    return BINE_OK;
+}
+
+// The device allocation holding [p, p + bytes) must contain the whole range.
+int check_range(const void* p, size_t bytes, const char* what) {
+    if (bytes == 0) return BINE_OK;
+    if (!p) return fail(BINE_ERR_ARG, std::string(what) + " is NULL");
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    hipError_t e = hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(BINE_ERR_ARG, std::string(what) + " is not device memory allocated with HIP");
+    }
+    if ((const char*)p + bytes > (const char*)base + size)
+        return fail(BINE_ERR_ARG, std::string(what) + " is smaller than P partitions of the per-rank size");
+    return BINE_OK;
+}
+
+bool overlap(const void* a, size_t na, const void* b, size_t nb) {
+    if (!a || !b || !na || !nb) return false;
+    const char* x = (const char*)a;
+    const char* y = (const char*)b;
+    return x < y + nb && y < x + na;
+}
+
+size_t partition_bytes(const DevPlan* dp, int buf) {
+    return (size_t)dp->cp.extent[buf] * dp->ds * (dp->cp.single[buf] ? 1 : dp->cp.P);
+}
+
+BufTable make_table(const DevPlan* dp, const void* send, void* recv) {
+    BufTable t{};
+    const void* bases[4] = {send, recv, dp->tmp[0], dp->tmp[1]};
+    for (int b = 0; b < 4; ++b) {
+        t.base[b] = (uint64_t)(uintptr_t)bases[b];
+        t.stride[b] = dp->cp.single[b] ? 0 : (uint64_t)dp->cp.extent[b] * dp->ds;
+    }
+    return t;
+}
+
+uint32_t vec_ok(const BufTable& t) {
+    for (int b = 0; b < 4; ++b)
+        if ((t.base[b] & 15) || (t.stride[b] & 15)) return 0;
+    return 1;
+}
+
+int launch_arrays(const DevArrays& d, int dtype, const BufTable& tab, uint32_t allow_vec, hipStream_t st) {
+    for (const auto& l : d.low.launches) {
+        if (!l.second) continue;
+        hipError_t e = launch_gred(dtype, d.max_lg, d.items + l.first, l.second, d.pool, tab, nullptr, false,
+                                   allow_vec, d.shape, st);
+        if (e != hipSuccess) return fail(BINE_ERR_HIP, std::string("gather-reduce launch: ") + hipGetErrorString(e));
+    }
+    return BINE_OK;
+}
+
+// Run a collective call end to end.
+int run_coll(bine_vcomm_t comm, const CollRequest& req, int dtype, const void* send, void* recv, void* stream) {
+    if (!comm) return fail(BINE_ERR_ARG, "NULL vcomm");
+    DevPlan* dp = nullptr;
+    int rc = get_plan(comm, req, dtype, &dp);
+    if (rc) return rc;
+    const size_t sbytes = partition_bytes(dp, BUF_SEND), rbytes = partition_bytes(dp, BUF_RECV);
+    if ((rc = check_range(send, sbytes, "sendbuf"))) return rc;
+    if ((rc = check_range(recv, rbytes, "recvbuf"))) return rc;
+    int exec = comm->cfg.exec;
+    const bool alias = overlap(send, sbytes, recv, rbytes);
+    if (alias) {
+        if (!(req.coll == BINE_COLL_ALLREDUCE && send == recv && sbytes == rbytes))
+            return fail(BINE_ERR_ARG, "sendbuf and recvbuf overlap");
+        if (exec == BINE_EXEC_FUSED) return fail(BINE_ERR_UNSUPPORTED, "fused execution cannot run in place");
+        exec = BINE_EXEC_STEP;
+    }
+    if (exec == BINE_EXEC_FUSED && !dp->cp.fused_ok)
+        return fail(BINE_ERR_UNSUPPORTED, "schedule cannot be fused: " + dp->cp.fused_why);
+    if (exec == BINE_EXEC_AUTO) exec = dp->cp.fused_ok ? BINE_EXEC_FUSED : BINE_EXEC_STEP;
+    HIPCHK(hipSetDevice(comm->cfg.device));
+    BufTable tab = make_table(dp, send, recv);
+    const uint32_t av = vec_ok(tab);
+    return launch_arrays(exec == BINE_EXEC_FUSED ? dp->fused : dp->step, dtype, tab, av, (hipStream_t)stream);
+}
+
+int parse_family(const char* s, int* out) {
+    if (!strcmp(s, "DEFAULT")) *out = BINE_FAMILY_DEFAULT;
+    else if (!strcmp(s, "BINE")) *out = BINE_FAMILY_BINE;
+    else if (!strcmp(s, "RECDOUB")) *out = BINE_FAMILY_RECDOUB;
+    else if (!strcmp(s, "BRUCK")) *out = BINE_FAMILY_BRUCK;
+    else if (!strcmp(s, "RING")) *out = BINE_FAMILY_RING;
+    else return BINE_ERR_ARG;
+    return BINE_OK;
+}
+
+struct AlgoName {
+    const char* name;
+    int value;
+};
+
+}  // namespace
+
+// ============================================================================
+// vcomm
+// ============================================================================
+
+extern "C" int bine_vcomm_config_init(bine_vcomm_config_t* cfg, uint32_t size) {
+    if (!cfg || size < 1) return fail(BINE_ERR_ARG, "bad config or size");
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->dimensions[0] = size;
+    cfg->dimensions[1] = cfg->dimensions[2] = 1;
+    cfg->dimensions_num = 1;
+    cfg->num_ports = 1;
+    bine_coll_config_t* all[8] = {&cfg->allreduce, &cfg->allgather, &cfg->reduce_scatter, &cfg->bcast,
+                                  &cfg->alltoall, &cfg->scatter, &cfg->gather, &cfg->reduce};
+    for (auto* c : all) {
+        c->family = BINE_FAMILY_BINE;
+        c->algo = 0;
+        c->distance = BINE_DISTANCE_INCREASING;
+    }
+    cfg->allreduce.algo = BINE_ALLREDUCE_B;
+    cfg->allgather.algo = BINE_ALLGATHER_CONT_PERMUTE;
+    cfg->allgather.distance = BINE_DISTANCE_DECREASING;
+    cfg->bcast.distance = BINE_DISTANCE_DECREASING;
+    cfg->gather.distance = BINE_DISTANCE_DECREASING;
+    cfg->exec = BINE_EXEC_AUTO;
+    cfg->device = 0;
+    return BINE_OK;
+}
+
+extern "C" int bine_vcomm_config_from_env(bine_vcomm_config_t* cfg) {
+    if (!cfg) return fail(BINE_ERR_ARG, "NULL config");
+    const char* s;
+    if ((s = getenv("LIBBINE_DIMENSIONS"))) {
+        uint32_t d[3] = {1, 1, 1}, n = 0;
+        std::string str(s);
+        size_t pos = 0;
+        while (pos <= str.size() && n < 4) {
+            size_t x = str.find('x', pos);
+            std::string tok = str.substr(pos, x == std::string::npos ? std::string::npos : x - pos);
+            if (tok.empty() || n == 3) return fail(BINE_ERR_ARG, "bad LIBBINE_DIMENSIONS");
+            d[n++] = (uint32_t)atoi(tok.c_str());
+            if (x == std::string::npos) break;
+            pos = x + 1;
+        }
+        for (uint32_t i = 0; i < 3; ++i) cfg->dimensions[i] = d[i];
+        cfg->dimensions_num = n;
+    }
+    if ((s = getenv("LIBBINE_NUM_PORTS"))) cfg->num_ports = (uint32_t)atoi(s);
+    if ((s = getenv("LIBBINE_EXEC"))) {
+        if (!strcmp(s, "AUTO")) cfg->exec = BINE_EXEC_AUTO;
+        else if (!strcmp(s, "STEP")) cfg->exec = BINE_EXEC_STEP;
+        else if (!strcmp(s, "FUSED")) cfg->exec = BINE_EXEC_FUSED;
+        else return fail(BINE_ERR_ARG, "bad LIBBINE_EXEC");
+    }
+    struct CollEnv {
+        const char* name;
+        bine_coll_config_t* c;
+        std::vector<AlgoName> algos;
+    } colls[] = {
+        {"ALLREDUCE", &cfg->allreduce, {{"L", 0}, {"B", 1}, {"REDUCE_BCAST", 2}, {"B_CONT", 3}, {"B_COALESCE", 4}}},
+        {"ALLGATHER", &cfg->allgather,
+         {{"VEC_DOUBLING_CONT_PERMUTE", 0}, {"VEC_DOUBLING_CONT_SEND", 1}, {"VEC_DOUBLING_BLOCKS", 2}, {"GATHER_BCAST", 3}}},
+        {"REDUCE_SCATTER", &cfg->reduce_scatter,
+         {{"VEC_HALVING_CONT_PERMUTE", 0}, {"VEC_HALVING_CONT_SEND", 1}, {"VEC_HALVING_BLOCKS", 2}, {"REDUCE_SCATTER", 3}}},
+        {"BCAST", &cfg->bcast, {{"BINOMIAL_TREE", 0}, {"BINOMIAL_TREE_TMPBUF", 1}, {"SCATTER_ALLGATHER", 2}}},
+        {"ALLTOALL", &cfg->alltoall, {{"LOG", 0}}},
+        {"SCATTER", &cfg->scatter, {{"BINOMIAL_TREE_CONT_PERMUTE", 0}, {"BINOMIAL_TREE_CONT_SEND", 1}}},
+        {"GATHER", &cfg->gather, {{"BINOMIAL_TREE_CONT_PERMUTE", 0}, {"BINOMIAL_TREE_CONT_SEND", 1}}},
+        {"REDUCE", &cfg->reduce, {{"BINOMIAL_TREE", 0}, {"REDUCE_SCATTER_GATHER", 1}}},
+    };
+    for (auto& ce : colls) {
+        std::string base = std::string("LIBBINE_") + ce.name;
+        if ((s = getenv((base + "_ALGO_FAMILY").c_str())) && parse_family(s, &ce.c->family))
+            return fail(BINE_ERR_ARG, "bad " + base + "_ALGO_FAMILY");
+        if ((s = getenv((base + "_ALGO").c_str()))) {
+            bool found = false;
+            for (auto& a : ce.algos)
+                if (!strcmp(a.name, s)) {
+                    ce.c->algo = a.value;
+                    found = true;
+                }
+            if (!found) return fail(BINE_ERR_ARG, "bad " + base + "_ALGO");
+        }
+        if ((s = getenv((base + "_DISTANCE").c_str())))
+            ce.c->distance = !strcmp(s, "INCREASING") ? BINE_DISTANCE_INCREASING : BINE_DISTANCE_DECREASING;
+    }
+    return BINE_OK;
+}
+
+extern "C" int bine_vcomm_create(const bine_vcomm_config_t* cfg, bine_vcomm_t* out) {
+    if (!cfg || !out) return fail(BINE_ERR_ARG, "NULL argument");
+    if (cfg->num_ports < 1 || cfg->num_ports > (uint32_t)kMaxPorts) return fail(BINE_ERR_ARG, "num_ports must be 1..6");
+    Torus t;
+    if (!Torus::make(cfg->dimensions, cfg->dimensions_num, &t)) return fail(BINE_ERR_ARG, "bad torus dimensions");
+    if (t.size < 2) return fail(BINE_ERR_ARG, "a virtual communicator needs at least two ranks");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return fail(BINE_ERR_NO_DEVICE, "no HIP device");
+    }
+    if (cfg->device < 0 || cfg->device >= ndev) return fail(BINE_ERR_ARG, "bad device ordinal");
+    auto* c = new bine_vcomm_s();
+    c->cfg = *cfg;
+    c->t = t;
+    *out = c;
+    return BINE_OK;
+}
+
+extern "C" int bine_vcomm_destroy(bine_vcomm_t comm) {
+    if (!comm) return BINE_OK;
+    (void)hipSetDevice(comm->cfg.device);
+    delete comm;
+    return BINE_OK;
+}
+
+extern "C" int bine_vcomm_size(bine_vcomm_t comm, uint32_t* size) {
+    if (!comm || !size) return fail(BINE_ERR_ARG, "NULL argument");
+    *size = comm->t.size;
+    return BINE_OK;
+}
+
+extern "C" int bine_vcomm_set_exec(bine_vcomm_t comm, int32_t exec) {
+    if (!comm || exec < 0 || exec > 2) return fail(BINE_ERR_ARG, "bad exec");
+    comm->cfg.exec = exec;
+    return BINE_OK;
+}
+
+// ============================================================================
+// collectives
+// ============================================================================
+
+extern "C" int bine_allreduce(const void* sendbuf, void* recvbuf, int count, bine_datatype_t datatype, bine_op_t op,
+                              bine_vcomm_t comm, void* stream) {
+    if (count < 0) return fail(BINE_ERR_ARG, "negative count");
+    CollRequest r{BINE_COLL_ALLREDUCE, count, (int)op, 0};
+    return run_coll(comm, r, datatype, sendbuf, recvbuf, stream);
+}
+
+extern "C" int bine_reduce_scatter(const void* sendbuf, void* recvbuf, const int recvcounts[], bine_datatype_t datatype,
+                                   bine_op_t op, bine_vcomm_t comm, void* stream) {
+    if (!comm || !recvcounts) return fail(BINE_ERR_ARG, "NULL argument");
+    for (uint32_t i = 1; i < comm->t.size; ++i)
+        if (recvcounts[i] != recvcounts[0])
+            return fail(BINE_ERR_UNSUPPORTED, "unequal recvcounts (the reference assumes a reduce_scatter_block)");
+    if (recvcounts[0] < 0) return fail(BINE_ERR_ARG, "negative count");
+    CollRequest r{BINE_COLL_REDUCE_SCATTER, recvcounts[0], (int)op, 0};
+    return run_coll(comm, r, datatype, sendbuf, recvbuf, stream);
+}
+
+extern "C" int bine_allgather(const void* sendbuf, int sendcount, bine_datatype_t sendtype, void* recvbuf,
+                              int recvcount, bine_datatype_t recvtype, bine_vcomm_t comm, void* stream) {
+    if (sendtype != recvtype || sendcount != recvcount)
+        return fail(BINE_ERR_UNSUPPORTED, "send and receive type/count must match (lib/libbine.cc:1170-1171)");
+    CollRequest r{BINE_COLL_ALLGATHER, sendcount, BINE_OP_SUM, 0};
+    return run_coll(comm, r, sendtype, sendbuf, recvbuf, stream);
+}
+
+extern "C" int bine_bcast(void* buffer, int count, bine_datatype_t datatype, int root, bine_vcomm_t comm, void* stream) {
+    if (count < 0) return fail(BINE_ERR_ARG, "negative count");
+    CollRequest r{BINE_COLL_BCAST, count, BINE_OP_SUM, root};
+    return run_coll(comm, r, datatype, nullptr, buffer, stream);
+}
+
+extern "C" int bine_alltoall(const void* sendbuf, int sendcount, bine_datatype_t sendtype, void* recvbuf, int recvcount,
+                             bine_datatype_t recvtype, bine_vcomm_t comm, void* stream) {
+    if (sendtype != recvtype || sendcount != recvcount)
+        return fail(BINE_ERR_UNSUPPORTED, "send and receive type/count must match");
+    CollRequest r{BINE_COLL_ALLTOALL, sendcount, BINE_OP_SUM, 0};
+    return run_coll(comm, r, sendtype, sendbuf, recvbuf, stream);
+}
+
+extern "C" int bine_scatter(const void* sendbuf, int sendcount, bine_datatype_t sendtype, void* recvbuf, int recvcount,
+                            bine_datatype_t recvtype, int root, bine_vcomm_t comm, void* stream) {
+    if (sendtype != recvtype || sendcount != recvcount)
+        return fail(BINE_ERR_UNSUPPORTED, "send and receive type/count must match (lib/libbine_coll_scatter.cc:195-196)");
+    CollRequest r{BINE_COLL_SCATTER, recvcount, BINE_OP_SUM, root};
+    return run_coll(comm, r, sendtype, sendbuf, recvbuf, stream);
+}
+
+extern "C" int bine_gather(const void* sendbuf, int sendcount, bine_datatype_t sendtype, void* recvbuf, int recvcount,
+                           bine_datatype_t recvtype, int root, bine_vcomm_t comm, void* stream) {
+    if (sendtype != recvtype || sendcount != recvcount)
+        return fail(BINE_ERR_UNSUPPORTED, "send and receive type/count must match (lib/libbine_coll_gather.cc:209)");
+    CollRequest r{BINE_COLL_GATHER, sendcount, BINE_OP_SUM, root};
+    return run_coll(comm, r, sendtype, sendbuf, recvbuf, stream);
+}
+
+extern "C" int bine_reduce(const void* sendbuf, void* recvbuf, int count, bine_datatype_t datatype, bine_op_t op, int root,
+                           bine_vcomm_t comm, void* stream) {
+    if (count < 0) return fail(BINE_ERR_ARG, "negative count");
+    CollRequest r{BINE_COLL_REDUCE, count, (int)op, root};
+    return run_coll(comm, r, datatype, sendbuf, recvbuf, stream);
+}
+
+extern "C" int bine_plan_stats(bine_vcomm_t comm, int32_t coll, int count, bine_datatype_t datatype, bine_op_t op,
+                               int root, int32_t exec, bine_plan_stats_t* out) {
+    if (!comm || !out) return fail(BINE_ERR_ARG, "NULL argument");
+    CollRequest r{coll, count, (int)op, root};
+    DevPlan* dp = nullptr;
+    int rc = get_plan(comm, r, datatype, &dp);
+    if (rc) return rc;
+    if (exec == BINE_EXEC_AUTO) exec = dp->cp.fused_ok ? BINE_EXEC_FUSED : BINE_EXEC_STEP;
+    if (exec == BINE_EXEC_FUSED && !dp->cp.fused_ok) return fail(BINE_ERR_UNSUPPORTED, "schedule cannot be fused: " + dp->cp.fused_why);
+    const DevArrays& d = exec == BINE_EXEC_FUSED ? dp->fused : dp->step;
+    out->launches = 0;
+    for (const auto& l : d.low.launches) out->launches += l.second ? 1 : 0;
+    out->items = d.low.items.size();
+    out->bytes_read = d.low.bytes_read;
+    out->bytes_written = d.low.bytes_written;
+    out->max_tree_leaves = d.low.max_leaves;
+    return BINE_OK;
+}
+
+namespace {
+
+// Issue one call of a collective (argument layout of its entry point).
+int issue(bine_vcomm_t comm, int32_t coll, const void* send, void* recv, int count, int dtype, int op, int root,
+          void* stream) {
+    switch (coll) {
+    case BINE_COLL_ALLREDUCE: return bine_allreduce(send, recv, count, (bine_datatype_t)dtype, (bine_op_t)op, comm, stream);
+    case BINE_COLL_REDUCE_SCATTER: {
+        std::vector<int> rc(comm->t.size, count);
+        return bine_reduce_scatter(send, recv, rc.data(), (bine_datatype_t)dtype, (bine_op_t)op, comm, stream);
+    }
+    case BINE_COLL_ALLGATHER:
+        return bine_allgather(send, count, (bine_datatype_t)dtype, recv, count, (bine_datatype_t)dtype, comm, stream);
+    case BINE_COLL_BCAST: return bine_bcast(recv, count, (bine_datatype_t)dtype, root, comm, stream);
+    case BINE_COLL_ALLTOALL:
+        return bine_alltoall(send, count, (bine_datatype_t)dtype, recv, count, (bine_datatype_t)dtype, comm, stream);
+    case BINE_COLL_SCATTER:
+        return bine_scatter(send, count, (bine_datatype_t)dtype, recv, count, (bine_datatype_t)dtype, root, comm, stream);
+    case BINE_COLL_GATHER:
+        return bine_gather(send, count, (bine_datatype_t)dtype, recv, count, (bine_datatype_t)dtype, root, comm, stream);
+    case BINE_COLL_REDUCE: return bine_reduce(send, recv, count, (bine_datatype_t)dtype, (bine_op_t)op, root, comm, stream);
+    default: return fail(BINE_ERR_ARG, "unknown collective");
+    }
+}
+
+}  // namespace
+
+extern "C" int bine_plan_autotune(bine_vcomm_t comm, int32_t coll, const void* sendbuf, void* recvbuf, int count,
+                                  bine_datatype_t datatype, bine_op_t op, int root, int32_t exec, void* stream,
+                                  int iters) {
+    if (!comm || iters < 1) return fail(BINE_ERR_ARG, "bad arguments");
+    CollRequest r{coll, count, (int)op, root};
+    DevPlan* dp = nullptr;
+    int rc = get_plan(comm, r, datatype, &dp);
+    if (rc) return rc;
+    if (exec == BINE_EXEC_AUTO) exec = dp->cp.fused_ok ? BINE_EXEC_FUSED : BINE_EXEC_STEP;
+    if (exec == BINE_EXEC_FUSED && !dp->cp.fused_ok) return fail(BINE_ERR_UNSUPPORTED, "schedule cannot be fused");
+    DevArrays& d = exec == BINE_EXEC_FUSED ? dp->fused : dp->step;
+    const std::vector<std::vector<Group>> one(1, dp->cp.fused);
+    const std::vector<std::vector<Group>>& groups = exec == BINE_EXEC_FUSED ? one : dp->cp.step_groups;
+    const int32_t saved_exec = comm->cfg.exec;
+    comm->cfg.exec = exec;
+    HIPCHK(hipSetDevice(comm->cfg.device));
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, comm->cfg.device) == hipSuccess) cus = prop.multiProcessorCount;
+    struct Cand {
+        int vpt, threads, persistent;
+    };
+    const Cand cands[] = {{1, 64, 0}, {1, 128, 0}, {1, 256, 0}, {2, 64, 0}, {2, 128, 0}, {2, 256, 0}, {4, 64, 0},
+                          {4, 128, 0}, {4, 256, 0}, {1, 512, 0}, {1, 256, 4 * cus}, {2, 256, 4 * cus}, {1, 64, 16 * cus}};
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    hipStream_t st = (hipStream_t)stream;
+    float best = 1e30f;
+    Cand bestc = {d.shape.vpt, d.shape.threads, d.shape.persistent};
+    int64_t best_tile = d.tile_elems;
+    rc = BINE_OK;
+    for (const Cand& c : cands) {
+        const int64_t tile = (int64_t)c.threads * c.vpt * (int64_t)(16 / dp->ds);
+        if (tile != d.tile_elems && (rc = upload(groups, dp->ds, tile, 0, &d))) break;
+        d.shape.vpt = c.vpt;
+        d.shape.threads = c.threads;
+        d.shape.persistent = c.persistent;
+        if ((rc = issue(comm, coll, sendbuf, recvbuf, count, datatype, op, root, stream))) break;
+        if (hipEventRecord(e0, st) != hipSuccess) { rc = fail(BINE_ERR_HIP, "event"); break; }
+        for (int i = 0; i < iters && !rc; ++i) rc = issue(comm, coll, sendbuf, recvbuf, count, datatype, op, root, stream);
+        if (rc) break;
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+            best = ms;
+            bestc = c;
+            best_tile = tile;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    comm->cfg.exec = saved_exec;
+    if (rc) return rc;
+    if (best_tile != d.tile_elems && (rc = upload(groups, dp->ds, best_tile, 0, &d))) return rc;
+    d.shape.vpt = bestc.vpt;
+    d.shape.threads = bestc.threads;
+    d.shape.persistent = bestc.persistent;
+    return BINE_OK;
+}
+
+extern "C" int bine_plan_dump(const bine_vcomm_config_t* cfg, int32_t coll, int count, bine_datatype_t datatype,
+                              bine_op_t op, int root, int32_t exec, int64_t tile_elems, bine_plan_dump_info_t* info,
+                              void* items, void* pool, uint32_t* launches) {
+    if (!cfg || !info) return fail(BINE_ERR_ARG, "NULL argument");
+    const size_t ds = dtsize(datatype);
+    if (!ds || count < 0) return fail(BINE_ERR_ARG, "bad datatype or count");
+    if (cfg->num_ports < 1 || cfg->num_ports > (uint32_t)kMaxPorts) return fail(BINE_ERR_ARG, "num_ports must be 1..6");
+    Torus t;
+    if (!Torus::make(cfg->dimensions, cfg->dimensions_num, &t) || t.size < 2) return fail(BINE_ERR_ARG, "bad torus");
+    CollRequest r{coll, count, (int)op, root};
+    Program prog;
+    CompiledPlan cp;
+    std::string err;
+    if (!build_program(t, *cfg, r, &prog, &err)) return fail(BINE_ERR_UNSUPPORTED, err);
+    if (!compile_program(prog, true, &cp, &err)) return fail(BINE_ERR_UNSUPPORTED, err);
+    if (exec == BINE_EXEC_AUTO) exec = cp.fused_ok ? BINE_EXEC_FUSED : BINE_EXEC_STEP;
+    if (exec == BINE_EXEC_FUSED && !cp.fused_ok) return fail(BINE_ERR_UNSUPPORTED, "schedule cannot be fused: " + cp.fused_why);
+    if (tile_elems <= 0) tile_elems = default_tile_elems(ds);
+    Lowered low;
+    if (exec == BINE_EXEC_FUSED) lower_groups(std::vector<std::vector<Group>>(1, cp.fused), tile_elems, (int)(16 / ds), 0, ds, &low);
+    else lower_groups(cp.step_groups, tile_elems, (int)(16 / ds), 0, ds, &low);
+    info->n_items = low.items.size();
+    info->n_pool = low.pool.size();
+    info->n_launches = low.launches.size();
+    for (int b = 0; b < 4; ++b) {
+        info->extent[b] = cp.extent[b];
+        info->single[b] = cp.single[b] ? 1 : 0;
+    }
+    info->P = cp.P;
+    info->fused_ok = cp.fused_ok ? 1 : 0;
+    if (items) std::memcpy(items, low.items.data(), low.items.size() * sizeof(GredItem));
+    if (pool) std::memcpy(pool, low.pool.data(), low.pool.size() * sizeof(PoolEntry));
+    if (launches)
+        for (size_t i = 0; i < low.launches.size(); ++i) {
+            launches[2 * i] = low.launches[i].first;
+            launches[2 * i + 1] = low.launches[i].second;
+        }
+    return BINE_OK;
+}
+
+// ============================================================================
+// schedule helpers
+// ============================================================================
+
+extern "C" int bine_compute_peers(const uint32_t* dims, uint32_t ndims, uint32_t rank, int port, int family,
+                                  uint32_t* peers) {
+    Torus t;
+    if (!dims || !peers || !Torus::make(dims, ndims, &t) || rank >= t.size || port < 0) return fail(BINE_ERR_ARG, "bad arguments");
+    if (family != FAM_BINE && family != FAM_RECDOUB) return fail(BINE_ERR_ARG, "family must be BINE or RECDOUB");
+    std::vector<uint32_t> v = peer_sequence(t, rank, (uint32_t)port, family);
+    std::copy(v.begin(), v.end(), peers);
+    return BINE_OK;
+}
+
+extern "C" int bine_compute_step_info(const uint32_t* dims, uint32_t ndims, uint32_t port, uint32_t* d, uint32_t* step_in_d) {
+    Torus t;
+    if (!dims || !d || !step_in_d || !Torus::make(dims, ndims, &t)) return fail(BINE_ERR_ARG, "bad arguments");
+    std::vector<StepDim> v = step_dims(t, port);
+    for (size_t i = 0; i < v.size(); ++i) {
+        d[i] = v[i].d;
+        step_in_d[i] = v[i].k;
+    }
+    return BINE_OK;
+}
+
+extern "C" int bine_block_step(const uint32_t* dims, uint32_t ndims, uint32_t rank, uint32_t port, int family,
+                               uint32_t* block_step) {
+    Torus t;
+    if (!dims || !block_step || !Torus::make(dims, ndims, &t) || rank >= t.size) return fail(BINE_ERR_ARG, "bad arguments");
+    if (family != FAM_BINE && family != FAM_RECDOUB) return fail(BINE_ERR_ARG, "family must be BINE or RECDOUB");
+    BlockSchedule bs;
+    bs.build(t, port, family);
+    for (uint32_t b = 0; b < t.size; ++b) block_step[b] = bs.send_step[(size_t)rank * t.size + b];
+    return BINE_OK;
+}
+
+extern "C" int bine_get_tree(const uint32_t* dims, uint32_t ndims, uint32_t root, uint32_t port, int family, int distance,
+                             uint32_t* parent, uint32_t* reached_at_step, uint32_t* remapped_ranks,
+                             uint32_t* remapped_ranks_max, uint32_t* subtree_roots) {
+    Torus t;
+    if (!dims || !Torus::make(dims, ndims, &t) || root >= t.size) return fail(BINE_ERR_ARG, "bad arguments");
+    Tree tr;
+    if (!build_tree(t, root, port, family, distance, &tr)) return fail(BINE_ERR_UNSUPPORTED, "tree numbering failed");
+    std::copy(tr.parent.begin(), tr.parent.end(), parent);
+    std::copy(tr.reached_at_step.begin(), tr.reached_at_step.end(), reached_at_step);
+    std::copy(tr.remapped.begin(), tr.remapped.end(), remapped_ranks);
+    std::copy(tr.remapped_max.begin(), tr.remapped_max.end(), remapped_ranks_max);
+    std::copy(tr.subtree_root.begin(), tr.subtree_root.end(), subtree_roots);
+    return BINE_OK;
+}
+
+extern "C" uint32_t bine_remap_rank(uint32_t rank, uint32_t size) { return remap_rank(rank, size); }
+
+extern "C" int bine_reduce_local(const void* inbuf, void* inoutbuf, int count, bine_datatype_t datatype, bine_op_t op,
+                                 void* stream) {
+    const size_t ds = dtsize(datatype);
+    if (!ds || count < 0) return fail(BINE_ERR_ARG, "bad arguments");
+    if (op == BINE_OP_VOID || count == 0) return BINE_OK;
+    int rc;
+    if ((rc = check_range(inbuf, (size_t)count * ds, "inbuf"))) return rc;
+    if ((rc = check_range(inoutbuf, (size_t)count * ds, "inoutbuf"))) return rc;
+    hipError_t e = launch_reduce_local(datatype, inbuf, inoutbuf, (uint64_t)count, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(BINE_ERR_HIP, hipGetErrorString(e));
+    return BINE_OK;
+}
+
+extern "C" const char* bine_last_error(void) { return g_err.c_str(); }
+
+// ============================================================================
+// batches
+// ============================================================================
+
+struct bine_batch_s {
+    bine_vcomm_t comm = nullptr;
+    std::vector<bine_batch_item_t> items;
+    std::vector<DevPlan*> plans;
+    std::vector<int> use_fused;  // per item
+    // one fused work list per element type
+    struct TypeList {
+        int dtype;
+        std::vector<std::vector<Group>> groups;  // one launch
+        std::vector<uint32_t> group_bucket;      // bucket of each group
+        DevArrays arr;
+    };
+    std::vector<TypeList> lists;
+    BufTable* d_tabs = nullptr;
+    std::vector<BufTable> tabs;
+    LaunchShape shape;
+    int64_t tile_bytes = 4096;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipStream_t graph_stream = nullptr;
+    ~bine_batch_s() {
+        for (auto& l : lists) l.arr.release();
+        if (d_tabs) (void)hipFree(d_tabs);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+namespace {
+
+int batch_lower(bine_batch_t b) {
+    for (auto& l : b->lists) {
+        l.arr.release();
+        const size_t ds = dtsize(l.dtype);
+        // lower each bucket separately so every item carries its bucket index
+        Lowered all;
+        int max_lg = 0;
+        for (size_t gi = 0; gi < l.groups[0].size(); ++gi) {
+            Lowered one;
+            std::vector<std::vector<Group>> v(1, std::vector<Group>{l.groups[0][gi]});
+            lower_groups(v, std::max<int64_t>(16 / (int64_t)ds, b->tile_bytes / (int64_t)ds), (int)(16 / ds),
+                         l.group_bucket[gi], ds, &one);
+            const uint32_t base = (uint32_t)all.pool.size();
+            for (auto it : one.items) {
+                it.leaf_off += base;
+                it.out_off += base;
+                all.items.push_back(it);
+            }
+            all.pool.insert(all.pool.end(), one.pool.begin(), one.pool.end());
+            all.bytes_read += one.bytes_read;
+            all.bytes_written += one.bytes_written;
+            all.max_leaves = std::max(all.max_leaves, one.max_leaves);
+            max_lg = std::max<int>(max_lg, l.groups[0][gi].lg);
+        }
+        all.launches.push_back({0, (uint32_t)all.items.size()});
+        l.arr.low = std::move(all);
+        l.arr.max_lg = max_lg;
+        if (!l.arr.low.items.empty()) {
+            HIPCHK(hipMalloc(&l.arr.items, l.arr.low.items.size() * sizeof(GredItem)));
+            HIPCHK(hipMemcpy(l.arr.items, l.arr.low.items.data(), l.arr.low.items.size() * sizeof(GredItem), hipMemcpyHostToDevice));
+            HIPCHK(hipMalloc(&l.arr.pool, l.arr.low.pool.size() * sizeof(PoolEntry)));
+            HIPCHK(hipMemcpy(l.arr.pool, l.arr.low.pool.data(), l.arr.low.pool.size() * sizeof(PoolEntry), hipMemcpyHostToDevice));
+        }
+    }
+    return BINE_OK;
+}
+
+int batch_run(bine_batch_t b, hipStream_t st) {
+    for (auto& l : b->lists) {
+        if (l.arr.low.items.empty()) continue;
+        hipError_t e = launch_gred(l.dtype, l.arr.max_lg, l.arr.items, (uint32_t)l.arr.low.items.size(), l.arr.pool,
+                                   BufTable{}, b->d_tabs, true, 1, b->shape, st);
+        if (e != hipSuccess) return fail(BINE_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
+    }
+    // collectives whose schedule cannot be fused run their step launches after
+    for (size_t i = 0; i < b->items.size(); ++i) {
+        if (b->use_fused[i]) continue;
+        DevPlan* dp = b->plans[i];
+        BufTable tab = b->tabs[i];
+        int rc = launch_arrays(dp->step, dp->dtype, tab, vec_ok(tab), st);
+        if (rc) return rc;
+    }
+    return BINE_OK;
+}
+
+}  // namespace
+
+extern "C" int bine_batch_create(bine_vcomm_t comm, const bine_batch_item_t* items, int n, bine_batch_t* out) {
+    if (!comm || !items || n <= 0 || !out) return fail(BINE_ERR_ARG, "bad arguments");
+    auto b = std::make_unique<bine_batch_s>();
+    b->comm = comm;
+    b->items.assign(items, items + n);
+    for (int i = 0; i < n; ++i) {
+        const bine_batch_item_t& it = items[i];
+        if (it.count < 0) return fail(BINE_ERR_ARG, "negative count");
+        CollRequest r{it.coll, it.count, it.op, it.root};
+        DevPlan* dp = nullptr;
+        int rc = get_plan(comm, r, it.datatype, &dp);
+        if (rc) return rc;
+        const void* send = it.coll == BINE_COLL_BCAST ? nullptr : it.sendbuf;
+        if ((rc = check_range(send, partition_bytes(dp, BUF_SEND), "batch sendbuf"))) return rc;
+        if ((rc = check_range(it.recvbuf, partition_bytes(dp, BUF_RECV), "batch recvbuf"))) return rc;
+        if (overlap(send, partition_bytes(dp, BUF_SEND), it.recvbuf, partition_bytes(dp, BUF_RECV)))
+            return fail(BINE_ERR_ARG, "batch item buffers overlap");
+        b->plans.push_back(dp);
+        b->tabs.push_back(make_table(dp, send, it.recvbuf));
+        const bool fuse = dp->cp.fused_ok && comm->cfg.exec != BINE_EXEC_STEP;
+        b->use_fused.push_back(fuse ? 1 : 0);
+        if (!fuse) continue;
+        auto lit = std::find_if(b->lists.begin(), b->lists.end(), [&](const bine_batch_s::TypeList& l) { return l.dtype == it.datatype; });
+        if (lit == b->lists.end()) {
+            b->lists.push_back(bine_batch_s::TypeList{it.datatype, std::vector<std::vector<Group>>(1), {}, DevArrays()});
+            lit = b->lists.end() - 1;
+        }
+        for (const Group& g : dp->cp.fused) {
+            lit->groups[0].push_back(g);
+            lit->group_bucket.push_back((uint32_t)i);
+        }
+    }
+    // buffers of different batch items must not overlap each other's outputs
+    for (size_t i = 0; i < b->items.size(); ++i)
+        for (size_t j = 0; j < b->items.size(); ++j) {
+            if (i == j) continue;
+            const DevPlan* pi = b->plans[i];
+            const DevPlan* pj = b->plans[j];
+            const void* si = b->items[i].coll == BINE_COLL_BCAST ? nullptr : b->items[i].sendbuf;
+            if (overlap(b->items[j].recvbuf, partition_bytes(pj, BUF_RECV), b->items[i].recvbuf, partition_bytes(pi, BUF_RECV)) ||
+                overlap(b->items[j].recvbuf, partition_bytes(pj, BUF_RECV), si, partition_bytes(pi, BUF_SEND)))
+                return fail(BINE_ERR_ARG, "a batch item's recvbuf overlaps another item's buffers");
+        }
+    HIPCHK(hipSetDevice(comm->cfg.device));
+    HIPCHK(hipMalloc(&b->d_tabs, sizeof(BufTable) * b->tabs.size()));
+    HIPCHK(hipMemcpy(b->d_tabs, b->tabs.data(), sizeof(BufTable) * b->tabs.size(), hipMemcpyHostToDevice));
+    int rc = batch_lower(b.get());
+    if (rc) return rc;
+    *out = b.release();
+    return BINE_OK;
+}
+
+extern "C" int bine_batch_launch(bine_batch_t b, void* stream) {
+    if (!b) return fail(BINE_ERR_ARG, "NULL batch");
+    HIPCHK(hipSetDevice(b->comm->cfg.device));
+    return batch_run(b, (hipStream_t)stream);
+}
+
+extern "C" int bine_batch_launch_graph(bine_batch_t b, void* stream) {
+    if (!b) return fail(BINE_ERR_ARG, "NULL batch");
+    HIPCHK(hipSetDevice(b->comm->cfg.device));
+    hipStream_t st = (hipStream_t)stream;
+    if (!b->gexec) {
+        hipStream_t cap;
+        HIPCHK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        HIPCHK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        int rc = batch_run(b, cap);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(cap, &g);
+        (void)hipStreamDestroy(cap);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(BINE_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+        b->graph = g;
+        HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(b->gexec, st));
+    return BINE_OK;
+}
+
+extern "C" int bine_batch_stats(bine_batch_t b, bine_plan_stats_t* out) {
+    if (!b || !out) return fail(BINE_ERR_ARG, "NULL argument");
+    std::memset(out, 0, sizeof(*out));
+    for (auto& l : b->lists) {
+        out->launches += l.arr.low.items.empty() ? 0 : 1;
+        out->items += l.arr.low.items.size();
+        out->bytes_read += l.arr.low.bytes_read;
+        out->bytes_written += l.arr.low.bytes_written;
+        out->max_tree_leaves = std::max(out->max_tree_leaves, l.arr.low.max_leaves);
+    }
+    for (size_t i = 0; i < b->items.size(); ++i) {
+        if (b->use_fused[i]) continue;
+        const DevArrays& d = b->plans[i]->step;
+        for (const auto& l : d.low.launches) out->launches += l.second ? 1 : 0;
+        out->items += d.low.items.size();
+        out->bytes_read += d.low.bytes_read;
+        out->bytes_written += d.low.bytes_written;
+    }
+    return BINE_OK;
+}
+
+extern "C" int bine_batch_get_tuning(bine_batch_t b, int32_t* vpt, int32_t* threads, int32_t* persistent_wgs) {
+    if (!b) return fail(BINE_ERR_ARG, "NULL batch");
+    if (vpt) *vpt = b->shape.vpt;
+    if (threads) *threads = b->shape.threads;
+    if (persistent_wgs) *persistent_wgs = b->shape.persistent;
+    return BINE_OK;
+}
+
+extern "C" int bine_batch_set_tuning(bine_batch_t b, int32_t vpt, int32_t threads, int32_t persistent_wgs) {
+    if (!b) return fail(BINE_ERR_ARG, "NULL batch");
+    if ((vpt != 1 && vpt != 2 && vpt != 4) || threads < 64 || threads > 512 || (threads & 63) || persistent_wgs < 0)
+        return fail(BINE_ERR_ARG, "bad launch shape");
+    b->shape.vpt = vpt;
+    b->shape.threads = threads;
+    b->shape.persistent = persistent_wgs;
+    const int64_t tb = (int64_t)threads * vpt * 16;
+    if (tb != b->tile_bytes) {
+        b->tile_bytes = tb;
+        if (b->gexec) {
+            (void)hipGraphExecDestroy(b->gexec);
+            (void)hipGraphDestroy(b->graph);
+            b->gexec = nullptr;
+            b->graph = nullptr;
+        }
+        return batch_lower(b);
+    }
+    if (b->gexec) {
+        (void)hipGraphExecDestroy(b->gexec);
+        (void)hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+    }
+    return BINE_OK;
+}
+
+extern "C" int bine_batch_autotune(bine_batch_t b, void* stream, int iters) {
+    if (!b || iters < 1) return fail(BINE_ERR_ARG, "bad arguments");
+    HIPCHK(hipSetDevice(b->comm->cfg.device));
+    hipStream_t st = (hipStream_t)stream;
+    struct Cand {
+        int vpt, threads, persistent;
+    };
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->comm->cfg.device) == hipSuccess) cus = prop.multiProcessorCount;
+    std::vector<Cand> cands = {{1, 64, 0},   {1, 128, 0},  {1, 256, 0}, {2, 128, 0},
+                               {2, 256, 0},  {4, 256, 0},  {1, 512, 0}, {2, 512, 0},
+                               {1, 256, 4 * cus}, {2, 256, 4 * cus}, {1, 128, 8 * cus}, {4, 256, 2 * cus}};
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    float best = 1e30f;
+    Cand bestc = {b->shape.vpt, b->shape.threads, b->shape.persistent};
+    for (const Cand& c : cands) {
+        int rc = bine_batch_set_tuning(b, c.vpt, c.threads, c.persistent);
+        if (rc) return rc;
+        rc = batch_run(b, st);  // warm-up
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(e0, st));
+        for (int i = 0; i < iters; ++i) {
+            rc = batch_run(b, st);
+            if (rc) return rc;
+        }
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            bestc = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return bine_batch_set_tuning(b, bestc.vpt, bestc.threads, bestc.persistent);
+}
+
+extern "C" int bine_batch_destroy(bine_batch_t b) {
+    if (!b) return BINE_OK;
+    (void)hipSetDevice(b->comm->cfg.device);
+    delete b;
+    return BINE_OK;
+}

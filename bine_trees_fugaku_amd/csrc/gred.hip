@@ -1,0 +1,273 @@
+// gred.hip -- the gather-reduce kernel that runs every Bine schedule on gfx950.
+//
+// A work item names a group of leaves (buffer, rank, offset) whose element-wise
+// reduction tree is perfect, a list of outputs, and a tile [delta, delta + n)
+// of elements.  A workgroup streams the tile: for each 16-byte vector it loads
+// the leaves in groups of eight, reduces each group pairwise in registers and
+// folds the group sums with a binary-counter stack, so the association is the
+// balanced tree ((l0+l1)+(l2+l3))+... over the leaf order the host chose.  The
+// sum is then stored to every output.  One-leaf items are copies (permutes),
+// two-leaf items are the per-step "received + local" reductions.
+//
+// Leaf and output entries are the same for the whole workgroup, so they are
+// fetched with scalar loads; data moves with 128-bit global loads/stores.
+// Compile with -Xclang -target-feature -Xclang -packed-fp32-ops.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "gred.hpp"
+
+namespace bine {
+
+template <typename T>
+struct Vec {
+    static constexpr int N = 16 / sizeof(T);
+    T v[N];
+};
+
+template <typename T>
+__device__ __forceinline__ T add1(T a, T b) {
+    return a + b;
+}
+template <>
+__device__ __forceinline__ int32_t add1<int32_t>(int32_t a, int32_t b) {
+    return (int32_t)((uint32_t)a + (uint32_t)b);
+}
+template <>
+__device__ __forceinline__ int64_t add1<int64_t>(int64_t a, int64_t b) {
+    return (int64_t)((uint64_t)a + (uint64_t)b);
+}
+template <>
+__device__ __forceinline__ int8_t add1<int8_t>(int8_t a, int8_t b) {
+    return (int8_t)((uint8_t)a + (uint8_t)b);
+}
+
+template <typename T>
+__device__ __forceinline__ Vec<T> vadd(const Vec<T>& a, const Vec<T>& b) {
+    Vec<T> r;
+#pragma unroll
+    for (int i = 0; i < Vec<T>::N; ++i) r.v[i] = add1<T>(a.v[i], b.v[i]);
+    return r;
+}
+
+template <typename T, bool NT>
+__device__ __forceinline__ Vec<T> vload(const char* p) {
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    u4 x;
+    if (NT) x = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p));
+    else x = *reinterpret_cast<const u4*>(p);
+    Vec<T> r;
+    __builtin_memcpy(&r, &x, 16);
+    return r;
+}
+
+template <typename T, bool NT>
+__device__ __forceinline__ void vstore(char* p, const Vec<T>& v) {
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    u4 x;
+    __builtin_memcpy(&x, &v, 16);
+    if (NT) __builtin_nontemporal_store(x, reinterpret_cast<u4*>(p));
+    else *reinterpret_cast<u4*>(p) = x;
+}
+
+// Buffer address of a (buffer, rank, element offset) entry.  The buffer is
+// picked with masks rather than an indexed or selected load so the table stays
+// in scalar registers.
+struct Table {
+    uint64_t b0, b1, b2, b3, s0, s1, s2, s3;
+    __device__ __forceinline__ char* at(uint32_t rb, uint64_t elem_off, uint32_t tsize) const {
+        const uint32_t buf = rb >> 28, rank = rb & 0x0FFFFFFFu;
+        const uint64_t m0 = 0ull - (uint64_t)(buf == 0), m1 = 0ull - (uint64_t)(buf == 1);
+        const uint64_t m2 = 0ull - (uint64_t)(buf == 2), m3 = 0ull - (uint64_t)(buf == 3);
+        const uint64_t base = (b0 & m0) | (b1 & m1) | (b2 & m2) | (b3 & m3);
+        const uint64_t stride = (s0 & m0) | (s1 & m1) | (s2 & m2) | (s3 & m3);
+        return reinterpret_cast<char*>(base + (uint64_t)rank * stride + elem_off * tsize);
+    }
+};
+
+// Evaluate the balanced tree over 2^lg leaves; `load(k)` fetches leaf k.
+template <typename W, int MAXS, typename AddF, typename LoadF>
+__device__ __forceinline__ W tree_eval(uint32_t lg, AddF add, LoadF load) {
+    if (lg == 0) return load(0);
+    if (lg == 1) return add(load(0), load(1));
+    if (lg == 2) return add(add(load(0), load(1)), add(load(2), load(3)));
+    W stk[MAXS];
+    const uint32_t S = lg - 3;
+    const uint32_t ngroups = 1u << S;
+    W res;
+    for (uint32_t j = 0; j < ngroups; ++j) {
+        const uint32_t k = j << 3;
+        W x0 = load(k + 0), x1 = load(k + 1), x2 = load(k + 2), x3 = load(k + 3);
+        W x4 = load(k + 4), x5 = load(k + 5), x6 = load(k + 6), x7 = load(k + 7);
+        W v = add(add(add(x0, x1), add(x2, x3)), add(add(x4, x5), add(x6, x7)));
+        // binary-counter carry: level l holds the sum of the previous 2^l groups
+        bool carry = true;
+#pragma unroll
+        for (int l = 0; l < MAXS; ++l) {
+            if (carry && l < (int)S) {
+                if ((j >> l) & 1u) {
+                    v = add(stk[l], v);
+                } else {
+                    stk[l] = v;
+                    carry = false;
+                }
+            }
+        }
+        res = v;
+    }
+    return res;
+}
+
+template <typename T, int VPT>
+struct Pack {
+    Vec<T> x[VPT];
+};
+
+template <typename T, int VPT, int MAXS, bool BATCH, bool NT>
+__global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ items, uint32_t nitems,
+                                                   const PoolEntry* __restrict__ pool, BufTable tab,
+                                                   const BufTable* __restrict__ tabs, uint32_t allow_vec) {
+    constexpr int VE = Vec<T>::N;
+    for (uint32_t ii = blockIdx.x; ii < nitems; ii += gridDim.x) {
+        const GredItem it = items[ii];
+        Table t;
+        if (BATCH) {
+            const BufTable& bt = tabs[it.bucket];
+            t = Table{bt.base[0], bt.base[1], bt.base[2], bt.base[3], bt.stride[0], bt.stride[1], bt.stride[2], bt.stride[3]};
+        } else {
+            t = Table{tab.base[0], tab.base[1], tab.base[2], tab.base[3], tab.stride[0], tab.stride[1], tab.stride[2], tab.stride[3]};
+        }
+        const PoolEntry* __restrict__ leaves = pool + it.leaf_off;
+        const PoolEntry* __restrict__ outs = pool + it.out_off;
+        const uint32_t lg = it.lg;
+        const uint64_t delta = it.delta;
+        if ((it.flags & ITEM_VEC) && allow_vec) {
+            const uint32_t nvec = it.n / VE;
+            for (uint32_t base = 0; base < nvec; base += blockDim.x * VPT) {
+                // lane-local vector indices of this pass
+                uint32_t vi[VPT];
+                bool ok[VPT];
+#pragma unroll
+                for (int u = 0; u < VPT; ++u) {
+                    vi[u] = base + threadIdx.x + u * blockDim.x;
+                    ok[u] = vi[u] < nvec;
+                }
+                auto addp = [](const Pack<T, VPT>& a, const Pack<T, VPT>& b) __attribute__((always_inline)) {
+                    Pack<T, VPT> r;
+#pragma unroll
+                    for (int u = 0; u < VPT; ++u) r.x[u] = vadd<T>(a.x[u], b.x[u]);
+                    return r;
+                };
+                auto loadp = [&](uint32_t k) __attribute__((always_inline)) {
+                    const PoolEntry e = leaves[k];
+                    const char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
+                    Pack<T, VPT> r;
+#pragma unroll
+                    for (int u = 0; u < VPT; ++u) {
+                        if (ok[u]) r.x[u] = vload<T, NT>(p + (uint64_t)vi[u] * 16);
+                        else r.x[u] = Vec<T>{};
+                    }
+                    return r;
+                };
+                Pack<T, VPT> s = tree_eval<Pack<T, VPT>, MAXS>(lg, addp, loadp);
+                for (uint32_t o = 0; o < it.nouts; ++o) {
+                    const PoolEntry e = outs[o];
+                    char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
+#pragma unroll
+                    for (int u = 0; u < VPT; ++u)
+                        if (ok[u]) vstore<T, NT>(p + (uint64_t)vi[u] * 16, s.x[u]);
+                }
+            }
+        } else {
+            for (uint32_t e0 = threadIdx.x; e0 < it.n; e0 += blockDim.x) {
+                auto adds = [](T a, T b) __attribute__((always_inline)) { return add1<T>(a, b); };
+                auto loads = [&](uint32_t k) __attribute__((always_inline)) {
+                    const PoolEntry e = leaves[k];
+                    return *reinterpret_cast<const T*>(t.at(e.rb, (uint64_t)e.off + delta + e0, sizeof(T)));
+                };
+                T s = tree_eval<T, MAXS>(lg, adds, loads);
+                for (uint32_t o = 0; o < it.nouts; ++o) {
+                    const PoolEntry e = outs[o];
+                    *reinterpret_cast<T*>(t.at(e.rb, (uint64_t)e.off + delta + e0, sizeof(T))) = s;
+                }
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ void reduce_local_kernel(const T* __restrict__ in, T* __restrict__ inout, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        inout[i] = add1<T>(in[i], inout[i]);
+}
+
+// ---- host-side dispatch ------------------------------------------------------
+
+template <typename T, int VPT, int MAXS, bool BATCH>
+static hipError_t launch_t(const GredItem* items, uint32_t n, const PoolEntry* pool, const BufTable& tab,
+                           const BufTable* tabs, uint32_t allow_vec, const LaunchShape& sh, hipStream_t st) {
+    uint32_t grid = sh.persistent > 0 ? (uint32_t)sh.persistent : n;
+    if (grid > n) grid = n;
+    if (grid == 0) return hipSuccess;
+    if (sh.nontemporal)
+        hipLaunchKernelGGL((gred_kernel<T, VPT, MAXS, BATCH, true>), dim3(grid), dim3(sh.threads), 0, st, items, n,
+                           pool, tab, tabs, allow_vec);
+    else
+        hipLaunchKernelGGL((gred_kernel<T, VPT, MAXS, BATCH, false>), dim3(grid), dim3(sh.threads), 0, st, items, n,
+                           pool, tab, tabs, allow_vec);
+    return hipGetLastError();
+}
+
+template <typename T, bool BATCH>
+static hipError_t launch_v(int max_lg, const GredItem* items, uint32_t n, const PoolEntry* pool, const BufTable& tab,
+                           const BufTable* tabs, uint32_t allow_vec, const LaunchShape& sh, hipStream_t st) {
+    const bool small = max_lg <= 9;
+    switch (sh.vpt) {
+    case 1:
+        return small ? launch_t<T, 1, 6, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st)
+                     : launch_t<T, 1, 11, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st);
+    case 2:
+        return small ? launch_t<T, 2, 6, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st)
+                     : launch_t<T, 2, 11, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st);
+    default:  // four vectors per thread only with the shallow stack (the deep one spills)
+        return small ? launch_t<T, 4, 6, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st)
+                     : launch_t<T, 2, 11, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st);
+    }
+}
+
+// dtype codes follow bine_datatype_t
+hipError_t launch_gred(int dtype, int max_lg, const GredItem* items, uint32_t n, const PoolEntry* pool,
+                       const BufTable& tab, const BufTable* tabs, bool batch, uint32_t allow_vec,
+                       const LaunchShape& sh, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+#define BINE_DISPATCH(T)                                                                               \
+    return batch ? launch_v<T, true>(max_lg, items, n, pool, tab, tabs, allow_vec, sh, st)             \
+                 : launch_v<T, false>(max_lg, items, n, pool, tab, tabs, allow_vec, sh, st)
+    switch (dtype) {
+    case 0: BINE_DISPATCH(int32_t);
+    case 1: BINE_DISPATCH(int64_t);
+    case 2: BINE_DISPATCH(float);
+    case 3: BINE_DISPATCH(double);
+    case 4: BINE_DISPATCH(int8_t);
+    default: return hipErrorInvalidValue;
+    }
+#undef BINE_DISPATCH
+}
+
+hipError_t launch_reduce_local(int dtype, const void* in, void* inout, uint64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    switch (dtype) {
+    case 0: hipLaunchKernelGGL(reduce_local_kernel<int32_t>, dim3(blocks), dim3(256), 0, st, (const int32_t*)in, (int32_t*)inout, n); break;
+    case 1: hipLaunchKernelGGL(reduce_local_kernel<int64_t>, dim3(blocks), dim3(256), 0, st, (const int64_t*)in, (int64_t*)inout, n); break;
+    case 2: hipLaunchKernelGGL(reduce_local_kernel<float>, dim3(blocks), dim3(256), 0, st, (const float*)in, (float*)inout, n); break;
+    case 3: hipLaunchKernelGGL(reduce_local_kernel<double>, dim3(blocks), dim3(256), 0, st, (const double*)in, (double*)inout, n); break;
+    case 4: hipLaunchKernelGGL(reduce_local_kernel<int8_t>, dim3(blocks), dim3(256), 0, st, (const int8_t*)in, (int8_t*)inout, n); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace bine

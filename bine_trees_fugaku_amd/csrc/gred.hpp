@@ -1,0 +1,47 @@
+// gred.hpp -- work-item format shared by the host plan compiler and the HIP
+// gather-reduce kernel.
+#pragma once
+
+#include <cstdint>
+
+namespace bine {
+
+// Location of one leaf or output: (buffer, rank, element offset).
+struct PoolEntry {
+    uint32_t rb;   // rank in bits 0..27, buffer id in bits 28..31
+    uint32_t off;  // element offset inside the rank's partition (at tile start minus delta)
+};
+
+inline PoolEntry make_entry(uint32_t buf, uint32_t rank, uint32_t off) { return PoolEntry{rank | (buf << 28), off}; }
+
+enum : uint16_t { ITEM_VEC = 1 };
+
+// One tile of one group: elements [delta, delta + n) of the group.
+struct alignas(16) GredItem {
+    uint32_t leaf_off;  // first leaf in the pool
+    uint32_t out_off;   // first output in the pool
+    uint16_t lg;        // log2(number of leaves)
+    uint16_t flags;     // ITEM_VEC: every leaf/output address is 16-byte aligned
+    uint32_t nouts;
+    uint32_t delta;
+    uint32_t n;
+    uint32_t bucket;    // buffer table index (batches); 0 for single calls
+    uint32_t pad;
+};
+static_assert(sizeof(GredItem) == 32, "GredItem layout");
+
+// Per-call buffer table: base address and per-rank stride of the four buffers.
+struct BufTable {
+    uint64_t base[4];
+    uint64_t stride[4];
+};
+
+// Launch shape of the gather-reduce kernel.
+struct LaunchShape {
+    int vpt = 1;          // 16-byte vectors per thread per pass: 1, 2 or 4
+    int threads = 256;    // workgroup size: 64..512
+    int persistent = 0;   // 0: one workgroup per item; >0: grid of that many workgroups
+    int nontemporal = 0;  // streaming loads/stores
+};
+
+}  // namespace bine

@@ -1,0 +1,233 @@
+/*
+ * bine_hip.h -- C ABI of libbine_hip.so: Bine-tree collectives over P virtual
+ * ranks that live as HBM partitions of one AMD Instinct MI355X (gfx950).
+ *
+ * Drop-in boundary for the MPI-layer data path of HLC-Lab/bine_trees_fugaku.
+ * Each collective keeps the argument order of the PMPI-interposed entry point
+ * it replaces (lib/libbine.cc), with three substitutions:
+ *   - MPI_Comm      -> bine_vcomm_t (a virtual communicator: torus dims, ports,
+ *                      per-collective algorithm family/variant/distance);
+ *   - MPI_Datatype  -> bine_datatype_t, MPI_Op -> bine_op_t (SUM, and the
+ *                      bench's no-op VOID operator, bench/bench.cc:240-256);
+ *   - a trailing `stream` (hipStream_t passed as void*; NULL = default stream).
+ *
+ * Buffer convention: a buffer argument is a device pointer to P consecutive
+ * per-rank partitions, rank r's partition at buf + r * (per-rank bytes), where
+ * the per-rank bytes are the MPI per-rank buffer size of that argument
+ * (e.g. count * sizeof(type) for MPI_Allreduce).  Arguments that MPI only reads
+ * or writes at the root (MPI_Scatter sendbuf, MPI_Gather recvbuf, MPI_Reduce
+ * recvbuf) are a single partition: the root's.
+ *
+ * All calls are stream-ordered and return a bine_status_t.  Preconditions the
+ * reference asserts (lib/libbine.cc:1035-1036, 1085, 148, lib/libbine_coll.cc:204)
+ * are reported as BINE_ERR_ARG / BINE_ERR_UNSUPPORTED instead of aborting.
+ * Schedules are compiled on the first call for a given (collective, count,
+ * type, op, root) and cached in the vcomm; a call with a cached schedule does
+ * not allocate and may be captured into a HIP graph.
+ */
+#ifndef BINE_HIP_H
+#define BINE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    BINE_OK = 0,
+    BINE_ERR_ARG = 1,         /* invalid argument / violated precondition */
+    BINE_ERR_UNSUPPORTED = 2, /* configuration the reference schedule does not realise */
+    BINE_ERR_HIP = 3,         /* HIP runtime error */
+    BINE_ERR_NOMEM = 4,
+    BINE_ERR_NO_DEVICE = 5
+} bine_status_t;
+
+/* MPI_INT32_T, MPI_INT64_T, MPI_FLOAT, MPI_DOUBLE, MPI_CHAR */
+typedef enum { BINE_INT32 = 0, BINE_INT64 = 1, BINE_FLOAT = 2, BINE_DOUBLE = 3, BINE_INT8 = 4 } bine_datatype_t;
+/* MPI_SUM, and the bench's VOID op (MPI_Reduce_local leaves inout unchanged) */
+typedef enum { BINE_OP_SUM = 0, BINE_OP_VOID = 1 } bine_op_t;
+/* bine_algo_family_t, lib/libbine_common.h:75-86 */
+typedef enum {
+    BINE_FAMILY_DEFAULT = 0, BINE_FAMILY_BINE = 1, BINE_FAMILY_RECDOUB = 2, BINE_FAMILY_BRUCK = 3, BINE_FAMILY_RING = 4
+} bine_family_t;
+/* bine_distance_type_t, lib/libbine_common.h:65-68 */
+typedef enum { BINE_DISTANCE_INCREASING = 0, BINE_DISTANCE_DECREASING = 1 } bine_distance_t;
+/* bine_allreduce_algo_t, lib/libbine_common.h:93-99 */
+typedef enum {
+    BINE_ALLREDUCE_L = 0, BINE_ALLREDUCE_B = 1, BINE_ALLREDUCE_REDUCE_BCAST = 2, BINE_ALLREDUCE_B_CONT = 3,
+    BINE_ALLREDUCE_B_COALESCE = 4
+} bine_allreduce_algo_t;
+/* bine_allgather_algo_t, lib/libbine_common.h:101-106 */
+typedef enum {
+    BINE_ALLGATHER_CONT_PERMUTE = 0, BINE_ALLGATHER_CONT_SEND = 1, BINE_ALLGATHER_BLOCKS = 2,
+    BINE_ALLGATHER_GATHER_BCAST = 3
+} bine_allgather_algo_t;
+/* bine_bcast_algo_t, lib/libbine_common.h:115-119 */
+typedef enum { BINE_BCAST_BINOMIAL_TREE = 0, BINE_BCAST_BINOMIAL_TREE_TMPBUF = 1, BINE_BCAST_SCATTER_ALLGATHER = 2 } bine_bcast_algo_t;
+/* bine_reduce_algo_t, lib/libbine_common.h:137-140 */
+typedef enum { BINE_REDUCE_BINOMIAL_TREE = 0, BINE_REDUCE_REDUCE_SCATTER_GATHER = 1 } bine_reduce_algo_t;
+
+/* How a collective executes on the GPU.  Both produce bit-identical results:
+ * STEP launches one gather-reduce kernel per schedule step (the reference's
+ * message structure); FUSED compiles the whole schedule into one launch that
+ * reads every input once and writes every output once, replaying each output
+ * element's reduction tree in the reference's association order. */
+typedef enum { BINE_EXEC_AUTO = 0, BINE_EXEC_STEP = 1, BINE_EXEC_FUSED = 2 } bine_exec_t;
+
+/* One collective's configuration: bine_*_config_t (lib/libbine_common.h:142-196).
+ * `algo` takes the collective's own enum above. */
+typedef struct {
+    int32_t family;   /* bine_family_t */
+    int32_t algo;     /* per-collective algorithm enum */
+    int32_t distance; /* bine_distance_t */
+} bine_coll_config_t;
+
+/* bine_env_t (lib/libbine_common.h:246-263) minus the uTofu/prealloc fields. */
+typedef struct {
+    uint32_t dimensions[3];
+    uint32_t dimensions_num;
+    uint32_t num_ports;
+    bine_coll_config_t allreduce, allgather, reduce_scatter, bcast, alltoall, scatter, gather, reduce;
+    int32_t exec; /* bine_exec_t */
+    int32_t device; /* HIP device ordinal the vcomm runs on */
+} bine_vcomm_config_t;
+
+typedef struct bine_vcomm_s* bine_vcomm_t;
+
+/* Defaults of init_env (lib/libbine.cc:28-73): 1D torus of `size` ranks, one
+ * port, family BINE, allreduce B, allgather CONT_PERMUTE + DECREASING, bcast
+ * and gather DECREASING, everything else INCREASING; exec AUTO, device 0. */
+int bine_vcomm_config_init(bine_vcomm_config_t* cfg, uint32_t size);
+/* Overlay the LIBBINE_* environment variables read by read_env (lib/libbine.cc:75-575):
+ * LIBBINE_DIMENSIONS, LIBBINE_NUM_PORTS, LIBBINE_<COLL>_ALGO_FAMILY / _ALGO / _DISTANCE,
+ * plus LIBBINE_EXEC = AUTO|STEP|FUSED. */
+int bine_vcomm_config_from_env(bine_vcomm_config_t* cfg);
+/* Replaces `new BineCommon(comm, env)` (lib/libbine_common.cc:116-179). */
+int bine_vcomm_create(const bine_vcomm_config_t* cfg, bine_vcomm_t* out);
+int bine_vcomm_destroy(bine_vcomm_t comm);
+/* MPI_Comm_size on the virtual communicator. */
+int bine_vcomm_size(bine_vcomm_t comm, uint32_t* size);
+/* Replace the execution mode of an existing vcomm (cached schedules are kept). */
+int bine_vcomm_set_exec(bine_vcomm_t comm, int32_t exec);
+
+/* MPI_Allreduce, lib/libbine.cc:1012 */
+int bine_allreduce(const void* sendbuf, void* recvbuf, int count, bine_datatype_t datatype, bine_op_t op,
+                   bine_vcomm_t comm, void* stream);
+/* MPI_Reduce_scatter, lib/libbine.cc:1064 (equal recvcounts, as the reference assumes :1065) */
+int bine_reduce_scatter(const void* sendbuf, void* recvbuf, const int recvcounts[], bine_datatype_t datatype,
+                        bine_op_t op, bine_vcomm_t comm, void* stream);
+/* MPI_Allgather, lib/libbine.cc:1168 */
+int bine_allgather(const void* sendbuf, int sendcount, bine_datatype_t sendtype, void* recvbuf, int recvcount,
+                   bine_datatype_t recvtype, bine_vcomm_t comm, void* stream);
+/* MPI_Bcast_f, lib/libbine.cc:1252 */
+int bine_bcast(void* buffer, int count, bine_datatype_t datatype, int root, bine_vcomm_t comm, void* stream);
+/* MPI_Alltoall, lib/libbine.cc:1291 */
+int bine_alltoall(const void* sendbuf, int sendcount, bine_datatype_t sendtype, void* recvbuf, int recvcount,
+                  bine_datatype_t recvtype, bine_vcomm_t comm, void* stream);
+/* MPI_Scatter, lib/libbine.cc:1312 */
+int bine_scatter(const void* sendbuf, int sendcount, bine_datatype_t sendtype, void* recvbuf, int recvcount,
+                 bine_datatype_t recvtype, int root, bine_vcomm_t comm, void* stream);
+/* MPI_Gather, lib/libbine.cc:1376 */
+int bine_gather(const void* sendbuf, int sendcount, bine_datatype_t sendtype, void* recvbuf, int recvcount,
+                bine_datatype_t recvtype, int root, bine_vcomm_t comm, void* stream);
+/* MPI_Reduce, lib/libbine.cc:1440 */
+int bine_reduce(const void* sendbuf, void* recvbuf, int count, bine_datatype_t datatype, bine_op_t op, int root,
+                bine_vcomm_t comm, void* stream);
+
+/* ---- schedule helpers (lib/libbine_coll.h:8-33), plain C for golden-vector tests ---- */
+/* compute_peers, lib/libbine_coll.cc:443: peers[num_steps] */
+int bine_compute_peers(const uint32_t* dims, uint32_t ndims, uint32_t rank, int port, int family, uint32_t* peers);
+/* compute_step_info, lib/libbine_coll.cc:306: d[num_steps], step_in_d[num_steps] */
+int bine_compute_step_info(const uint32_t* dims, uint32_t ndims, uint32_t port, uint32_t* d, uint32_t* step_in_d);
+/* BineBitmapCalculator::block_step, lib/libbine_common.cc:1491-1510, 2036-2045: block_step[P] */
+int bine_block_step(const uint32_t* dims, uint32_t ndims, uint32_t rank, uint32_t port, int family,
+                    uint32_t* block_step);
+/* get_tree, lib/libbine_coll.cc:334: five arrays of P entries */
+int bine_get_tree(const uint32_t* dims, uint32_t ndims, uint32_t root, uint32_t port, int family, int distance,
+                  uint32_t* parent, uint32_t* reached_at_step, uint32_t* remapped_ranks,
+                  uint32_t* remapped_ranks_max, uint32_t* subtree_roots);
+/* remap_rank, lib/libbine_coll_alltoall.cc:372 */
+uint32_t bine_remap_rank(uint32_t rank, uint32_t size);
+/* reduce_local, lib/libbine_coll.cc:153 (2-argument form: inout = in + inout), on device */
+int bine_reduce_local(const void* inbuf, void* inoutbuf, int count, bine_datatype_t datatype, bine_op_t op,
+                      void* stream);
+
+/* ---- introspection: bytes a call moves (for roofline accounting) ---- */
+typedef struct {
+    uint64_t launches;        /* kernel launches per call */
+    uint64_t items;           /* gather-reduce work items over all launches */
+    uint64_t bytes_read;      /* HBM bytes read by the kernels (schedule bytes) */
+    uint64_t bytes_written;   /* HBM bytes written by the kernels */
+    uint64_t max_tree_leaves; /* widest reduction tree replayed by one item */
+} bine_plan_stats_t;
+
+/* Collective codes for plan queries and batches. */
+typedef enum {
+    BINE_COLL_ALLREDUCE = 0, BINE_COLL_REDUCE_SCATTER = 1, BINE_COLL_ALLGATHER = 2, BINE_COLL_BCAST = 3,
+    BINE_COLL_ALLTOALL = 4, BINE_COLL_SCATTER = 5, BINE_COLL_GATHER = 6, BINE_COLL_REDUCE = 7
+} bine_coll_t;
+
+/* Compile (or fetch) the schedule of a collective without running it. */
+int bine_plan_stats(bine_vcomm_t comm, int32_t coll, int count, bine_datatype_t datatype, bine_op_t op, int root,
+                    int32_t exec, bine_plan_stats_t* out);
+
+/* Time candidate launch shapes (vectors per thread, workgroup size, persistent
+ * grid) of one collective's cached schedule on the caller's buffers and keep
+ * the fastest for later calls.  Runs the collective 13*(iters+1) times. */
+int bine_plan_autotune(bine_vcomm_t comm, int32_t coll, const void* sendbuf, void* recvbuf, int count,
+                       bine_datatype_t datatype, bine_op_t op, int root, int32_t exec, void* stream, int iters);
+
+/* Host-only introspection: compile a schedule without a GPU and return its
+ * lowered work (the arrays one call would upload).  Call once with NULL
+ * arrays to size them.  items: info->n_items records of 32 bytes (bine
+ * gather-reduce items); pool: info->n_pool records of 8 bytes; launches:
+ * info->n_launches pairs (first item, item count).  No HIP call is made. */
+typedef struct {
+    uint64_t n_items, n_pool, n_launches;
+    int64_t extent[4];  /* per-rank elements of SEND, RECV, TMP, TMP2 */
+    int32_t single[4];  /* buffer is one partition (the root's) */
+    uint32_t P;
+    int32_t fused_ok;
+} bine_plan_dump_info_t;
+int bine_plan_dump(const bine_vcomm_config_t* cfg, int32_t coll, int count, bine_datatype_t datatype, bine_op_t op,
+                   int root, int32_t exec, int64_t tile_elems, bine_plan_dump_info_t* info, void* items, void* pool,
+                   uint32_t* launches);
+
+/* ---- batching: many collectives in one launch (no counterpart in the reference:
+ * it replaces the per-call read_env/BlockInfo/tmpbuf set-up, lib/libbine.cc:1013-1050,
+ * for callers that issue many buckets at once) ---- */
+typedef struct {
+    int32_t coll;        /* bine_coll_t */
+    const void* sendbuf; /* layout as in the single-call entry point */
+    void* recvbuf;
+    int32_t count;       /* count (allreduce, bcast, reduce) or per-rank block count (others) */
+    int32_t datatype;    /* bine_datatype_t */
+    int32_t op;          /* bine_op_t */
+    int32_t root;
+} bine_batch_item_t;
+
+typedef struct bine_batch_s* bine_batch_t;
+
+/* Compile `n` collectives on one vcomm into one fused work list. */
+int bine_batch_create(bine_vcomm_t comm, const bine_batch_item_t* items, int n, bine_batch_t* out);
+/* Run the whole batch: one persistent launch per element type present. */
+int bine_batch_launch(bine_batch_t batch, void* stream);
+/* Same work captured once into a HIP graph and replayed. */
+int bine_batch_launch_graph(bine_batch_t batch, void* stream);
+/* Time candidate (vector width, workgroup, persistence) shapes on the batch and keep the fastest. */
+int bine_batch_autotune(bine_batch_t batch, void* stream, int iters);
+int bine_batch_stats(bine_batch_t batch, bine_plan_stats_t* out);
+/* Chosen launch shape: vectors per thread, threads per workgroup, persistent grid (0 = one workgroup per item). */
+int bine_batch_get_tuning(bine_batch_t batch, int32_t* vpt, int32_t* threads, int32_t* persistent_wgs);
+int bine_batch_set_tuning(bine_batch_t batch, int32_t vpt, int32_t threads, int32_t persistent_wgs);
+int bine_batch_destroy(bine_batch_t batch);
+
+/* Last error message of the calling thread (static storage). */
+const char* bine_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BINE_HIP_H */

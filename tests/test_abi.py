@@ -1,0 +1,113 @@
+"""The C-ABI library loads on the CPU and exports every symbol include/bine_hip.h declares."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from bine_trees_fugaku_amd import _lib
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "bine_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?(?:int|uint32_t|void|const char\*|char\*)\s*\**\s*(bine_\w+)\s*\(", text, re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_the_mpi_entry_points():
+    names = declared_functions()
+    for n in ("bine_allreduce", "bine_reduce_scatter", "bine_allgather", "bine_bcast", "bine_alltoall",
+              "bine_scatter", "bine_gather", "bine_reduce"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+    assert sorted(_lib.EXPORTED) == declared_functions()
+
+
+def test_config_defaults_follow_init_env():
+    cfg = _lib.VCommConfig()
+    _lib.check(_lib.load().bine_vcomm_config_init(ctypes.byref(cfg), 64), "init")
+    assert cfg.dimensions_num == 1 and cfg.dimensions[0] == 64 and cfg.num_ports == 1
+    assert cfg.allreduce.family == 1 and cfg.allreduce.algo == 1 and cfg.allreduce.distance == 0
+    assert cfg.allgather.algo == 0 and cfg.allgather.distance == 1  # CONT_PERMUTE, DECREASING
+    assert cfg.bcast.distance == 1 and cfg.gather.distance == 1
+
+
+def test_config_from_env(monkeypatch):
+    monkeypatch.setenv("LIBBINE_DIMENSIONS", "8x8x8")
+    monkeypatch.setenv("LIBBINE_NUM_PORTS", "3")
+    monkeypatch.setenv("LIBBINE_ALLREDUCE_ALGO", "B_CONT")
+    monkeypatch.setenv("LIBBINE_ALLTOALL_ALGO_FAMILY", "BRUCK")
+    monkeypatch.setenv("LIBBINE_EXEC", "STEP")
+    cfg = _lib.VCommConfig()
+    L = _lib.load()
+    _lib.check(L.bine_vcomm_config_init(ctypes.byref(cfg), 2), "init")
+    _lib.check(L.bine_vcomm_config_from_env(ctypes.byref(cfg)), "env")
+    assert list(cfg.dimensions) == [8, 8, 8] and cfg.dimensions_num == 3 and cfg.num_ports == 3
+    assert cfg.allreduce.algo == 3 and cfg.alltoall.family == 3 and cfg.exec == 1
+    monkeypatch.setenv("LIBBINE_ALLREDUCE_ALGO", "NOPE")
+    assert L.bine_vcomm_config_from_env(ctypes.byref(cfg)) == _lib.BINE_ERR_ARG
+
+
+def test_schedule_helpers_match_oracle():
+    from oracle import oracle as O
+    L = _lib.load()
+    for dims in ([8], [4, 4], [8, 8, 8], [2, 8, 2], [6, 6], [32, 256]):
+        d = (ctypes.c_uint32 * 3)(*(list(dims) + [1] * (3 - len(dims))))
+        P = 1
+        for x in dims:
+            P *= x
+        steps = len(O.step_info(dims))
+        ranks = range(P) if P <= 64 else [0, 1, 7, P // 2 + 3, P - 1]
+        for port in range(min(3, 2 * len(dims))):
+            sd = (ctypes.c_uint32 * steps)()
+            sk = (ctypes.c_uint32 * steps)()
+            _lib.check(L.bine_compute_step_info(d, len(dims), port, sd, sk), "step_info")
+            assert [(sd[i], sk[i]) for i in range(steps)] == O.step_info(dims, port)
+            for r in ranks:
+                out = (ctypes.c_uint32 * steps)()
+                _lib.check(L.bine_compute_peers(d, len(dims), r, port, 1, out), "peers")
+                assert list(out) == O.compute_peers(dims, r, port)
+                if P <= 512:
+                    bs = (ctypes.c_uint32 * P)()
+                    _lib.check(L.bine_block_step(d, len(dims), r, port, 1, bs), "block_step")
+                    assert list(bs) == list(O.block_step(dims, r, port))
+
+
+def test_trees_and_remap_match_oracle():
+    from oracle import oracle as O
+    L = _lib.load()
+    for dims in ([8], [16], [4, 4], [4, 4, 4]):
+        P = 1
+        for x in dims:
+            P *= x
+        d = (ctypes.c_uint32 * 3)(*(list(dims) + [1] * (3 - len(dims))))
+        for root in (0, 1, P - 1):
+            for dist in (0, 1):
+                arrs = [(ctypes.c_uint32 * P)() for _ in range(5)]
+                _lib.check(L.bine_get_tree(d, len(dims), root, 0, 1, dist, *arrs), "get_tree")
+                ref = O.get_tree(dims, root, dist=["INCREASING", "DECREASING"][dist])
+                for a, key in zip(arrs, ("parent", "reached_at_step", "remapped_ranks", "remapped_ranks_max",
+                                         "subtree_roots")):
+                    assert list(a) == list(ref[key]), (dims, root, dist, key)
+    for P in (2, 4, 8, 16, 64, 512, 6, 10, 12):
+        for r in range(min(P, 64)):
+            assert L.bine_remap_rank(r, P) == O.remap_rank(r, P)
+
+
+def test_no_device_is_reported_not_crashed():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    L = _lib.load()
+    cfg = _lib.VCommConfig()
+    _lib.check(L.bine_vcomm_config_init(ctypes.byref(cfg), 8), "init")
+    h = ctypes.c_void_p()
+    assert L.bine_vcomm_create(ctypes.byref(cfg), ctypes.byref(h)) == _lib.BINE_ERR_NO_DEVICE

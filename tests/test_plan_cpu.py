@@ -1,0 +1,232 @@
+"""CPU tests of the host schedule compiler of libbine_hip.so.
+
+Every schedule is compiled with ``plan_dump`` (no GPU involved), executed by
+the numpy model of the gather-reduce kernel (tests/emulator.py) and compared
+byte for byte with the CPU oracle, in both execution modes (STEP: one launch
+per schedule step; FUSED: one launch).  Only the HIP kernel itself is left for
+the GPU tests.
+"""
+import numpy as np
+import pytest
+
+from bine_trees_fugaku_amd import plan_dump
+from bine_trees_fugaku_amd._lib import BineError
+from oracle import oracle as O
+from tests import emulator as E
+
+NP = {"int32": np.int32, "int64": np.int64, "float32": np.float32, "float64": np.float64}
+
+
+def data(shape, dtype, seed):
+    rng = np.random.default_rng(seed)
+    if dtype.startswith("int"):
+        return rng.integers(-1000, 1001, size=shape).astype(NP[dtype])
+    return rng.standard_normal(shape).astype(NP[dtype])
+
+
+def same_bits(a, b):
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    return a.shape == b.shape and a.tobytes() == b.tobytes()
+
+
+def run_plan(dims, coll, count, dtype, exec, send=None, recv=None, tile=1 << 16, **kw):
+    meta, items, pool, launches = plan_dump(dims, coll, count, dtype, exec=exec, tile_elems=tile, **kw)
+    bufs = E.alloc_buffers(meta, NP[dtype], send, recv)
+    E.run(meta, items, pool, launches, bufs)
+    return meta, bufs
+
+
+def modes(dims, coll, count, dtype, **kw):
+    out = ["STEP"]
+    meta, *_ = plan_dump(dims, coll, count, dtype, exec="STEP", **kw)
+    if meta["fused_ok"]:
+        out.append("FUSED")
+    return out
+
+
+AR_TORI = [[8], [2, 2, 2], [4, 4], [4, 4, 4], [2, 8, 2], [16], [6], [10], [6, 6], [2, 6, 2], [4, 6, 4]]
+
+
+@pytest.mark.parametrize("dims", AR_TORI)
+@pytest.mark.parametrize("algo", ["B", "B_COALESCE", "L", "B_CONT"])
+@pytest.mark.parametrize("dtype", ["int32", "float32", "float64"])
+def test_allreduce(dims, algo, dtype):
+    P = int(np.prod(dims))
+    if algo == "B_CONT" and P & (P - 1):
+        pytest.skip("B_CONT is power-of-two only")
+    count = 3 * P + 5
+    x = data((P, count), dtype, P)
+    ref = O.allreduce(x, dims, algo=algo)
+    for ex in modes(dims, "allreduce", count, dtype, allreduce_algo=algo):
+        _, bufs = run_plan(dims, "allreduce", count, dtype, ex, send=x, allreduce_algo=algo)
+        assert same_bits(bufs[E.BUF_RECV], ref), (dims, algo, ex)
+
+
+def test_allreduce_fused_available_on_p2():
+    for dims in ([2, 2, 2], [4, 4, 4], [8, 8, 8]):
+        meta, *_ = plan_dump(dims, "allreduce", 4096 * 8, "float64", exec="STEP")
+        assert meta["fused_ok"]
+
+
+@pytest.mark.parametrize("ports", [2, 3, 6])
+@pytest.mark.parametrize("algo", ["B", "L"])
+def test_allreduce_multiport(ports, algo):
+    dims = [4, 4, 4]
+    count = 64 * ports + 11
+    x = data((64, count), "float32", ports)
+    ref = O.allreduce(x, dims, algo=algo, ports=ports)
+    for ex in modes(dims, "allreduce", count, "float32", num_ports=ports, allreduce_algo=algo):
+        _, bufs = run_plan(dims, "allreduce", count, "float32", ex, send=x, num_ports=ports, allreduce_algo=algo)
+        assert same_bits(bufs[E.BUF_RECV], ref)
+
+
+def test_allreduce_recdoub():
+    dims = [4, 4]
+    x = data((16, 100), "float64", 1)
+    ref = O.allreduce(x, dims, family="RECDOUB")
+    for ex in ("STEP", "FUSED"):
+        _, bufs = run_plan(dims, "allreduce", 100, "float64", ex, send=x, allreduce_family="RECDOUB")
+        assert same_bits(bufs[E.BUF_RECV], ref)
+
+
+def test_allreduce_void():
+    dims = [4, 4]
+    x = data((16, 48), "int32", 2)
+    ref = O.allreduce(x, dims, op="VOID")
+    for ex in ("STEP", "FUSED"):
+        _, bufs = run_plan(dims, "allreduce", 48, "int32", ex, send=x, op="VOID")
+        assert same_bits(bufs[E.BUF_RECV], ref)
+
+
+def test_allreduce_preconditions():
+    with pytest.raises(BineError):  # count < ports * P (lib/libbine.cc:1036)
+        plan_dump([4, 4], "allreduce", 10, "int32")
+    with pytest.raises(BineError):  # B needs INCREASING distance (lib/libbine.cc:1035)
+        plan_dump([4, 4], "allreduce", 64, "int32", allreduce_distance="DECREASING")
+    with pytest.raises(BineError):  # B_CONT on non power of two
+        plan_dump([6], "allreduce", 64, "int32", allreduce_algo="B_CONT")
+
+
+@pytest.mark.parametrize("dims", [[8], [4, 4], [2, 2, 2], [4, 4, 4], [16]])
+@pytest.mark.parametrize("dtype", ["int32", "float32", "float64"])
+def test_reduce_scatter(dims, dtype):
+    P = int(np.prod(dims))
+    c = 5
+    x = data((P, c * P), dtype, P)
+    ref = O.reduce_scatter(x, dims)
+    for ex in modes(dims, "reduce_scatter", c, dtype):
+        _, bufs = run_plan(dims, "reduce_scatter", c, dtype, ex, send=x)
+        assert same_bits(bufs[E.BUF_RECV], ref), ex
+
+
+@pytest.mark.parametrize("dims", [[8], [4, 4], [2, 2, 2], [4, 4, 4], [16]])
+@pytest.mark.parametrize("algo", ["VEC_DOUBLING_CONT_PERMUTE", "VEC_DOUBLING_CONT_SEND", "VEC_DOUBLING_BLOCKS"])
+def test_allgather(dims, algo):
+    if algo == "VEC_DOUBLING_BLOCKS" and len(dims) > 1:
+        pytest.skip("1D only")
+    P = int(np.prod(dims))
+    a = data((P, 7), "int32", P)
+    variant = {"VEC_DOUBLING_CONT_PERMUTE": "CONT_PERMUTE", "VEC_DOUBLING_CONT_SEND": "CONT_SEND",
+               "VEC_DOUBLING_BLOCKS": "BLOCKS"}[algo]
+    ref = O.allgather(a, dims, variant=variant)
+    for ex in modes(dims, "allgather", 7, "int32", allgather_algo=algo):
+        _, bufs = run_plan(dims, "allgather", 7, "int32", ex, send=a, allgather_algo=algo)
+        assert same_bits(bufs[E.BUF_RECV], ref), ex
+
+
+def test_allgather_multiport():
+    a = data((64, 12), "float32", 4)
+    ref = O.allgather(a, [4, 4, 4], ports=3)
+    for ex in ("STEP", "FUSED"):
+        _, bufs = run_plan([4, 4, 4], "allgather", 12, "float32", ex, send=a, num_ports=3)
+        assert same_bits(bufs[E.BUF_RECV], ref)
+
+
+@pytest.mark.parametrize("P", [2, 4, 8, 16, 64])
+@pytest.mark.parametrize("family", ["BINE", "BRUCK"])
+def test_alltoall(P, family):
+    c = 3
+    x = data((P, c * P), "int64", P)
+    ref = O.alltoall(x, P, family=family)
+    for ex in modes([P], "alltoall", c, "int64", alltoall_family=family):
+        _, bufs = run_plan([P], "alltoall", c, "int64", ex, send=x, alltoall_family=family)
+        assert same_bits(bufs[E.BUF_RECV], ref), ex
+
+
+@pytest.mark.parametrize("P", [6, 10, 12])
+def test_alltoall_bruck_non_p2(P):
+    x = data((P, 2 * P), "int32", P)
+    ref = O.alltoall(x, P, family="BRUCK")
+    for ex in ("STEP", "FUSED"):
+        _, bufs = run_plan([P], "alltoall", 2, "int32", ex, send=x, alltoall_family="BRUCK")
+        assert same_bits(bufs[E.BUF_RECV], ref)
+
+
+@pytest.mark.parametrize("P", [2, 4, 8, 16, 64])
+def test_bcast(P):
+    for root in sorted({0, 1, P // 2, P - 1}):
+        b = np.zeros((P, 9), np.float32)
+        b[root] = data((9,), "float32", root)
+        ref = O.bcast(b, root)
+        for ex in modes([P], "bcast", 9, "float32", root=root):
+            _, bufs = run_plan([P], "bcast", 9, "float32", ex, recv=b, root=root)
+            assert same_bits(bufs[E.BUF_RECV], ref), (root, ex)
+    b = np.zeros((P, 13), np.int32)
+    b[0] = data((13,), "int32", 5)
+    ref = O.bcast(b, 0, "SCATTER_ALLGATHER")
+    for ex in ("STEP", "FUSED"):
+        _, bufs = run_plan([P], "bcast", 13, "int32", ex, recv=b, bcast_algo="SCATTER_ALLGATHER",
+                           bcast_distance="INCREASING")
+        assert same_bits(bufs[E.BUF_RECV], ref)
+
+
+@pytest.mark.parametrize("P", [2, 4, 8, 16, 64])
+@pytest.mark.parametrize("dtype", ["int32", "float64"])
+def test_reduce(P, dtype):
+    for root in sorted({0, 1, P - 1}):
+        s = data((P, 9), dtype, root + P)
+        ref = O.reduce(s, root)
+        for ex in modes([P], "reduce", 9, dtype, root=root):
+            _, bufs = run_plan([P], "reduce", 9, dtype, ex, send=s, root=root)
+            assert same_bits(bufs[E.BUF_RECV][0], ref), (root, ex)
+    s = data((P, 13), dtype, 3)
+    ref = O.reduce(s, 0, "REDUCE_SCATTER_GATHER")
+    for ex in ("STEP", "FUSED"):
+        _, bufs = run_plan([P], "reduce", 13, dtype, ex, send=s, reduce_algo="REDUCE_SCATTER_GATHER")
+        assert same_bits(bufs[E.BUF_RECV][0], ref)
+
+
+@pytest.mark.parametrize("P", [2, 4, 8, 16, 64])
+def test_scatter_gather(P):
+    sr = data((P * 4,), "int32", P)
+    ref = O.scatter(sr, P, 0)
+    for ex in ("STEP", "FUSED"):
+        _, bufs = run_plan([P], "scatter", 4, "int32", ex, send=sr, root=0)
+        assert same_bits(bufs[E.BUF_RECV], ref)
+    s = data((P, 4), "int32", P + 1)
+    for root in range(0, P, 2):
+        ref = O.gather(s, root)
+        for ex in ("STEP", "FUSED"):
+            _, bufs = run_plan([P], "gather", 4, "int32", ex, send=s, root=root)
+            assert same_bits(bufs[E.BUF_RECV][0], ref), (root, ex)
+
+
+def test_unsupported_configurations_are_errors():
+    with pytest.raises(BineError):
+        plan_dump([4], "gather", 4, "int32", root=1)  # odd root: wrong data in the reference
+    with pytest.raises(BineError):
+        plan_dump([6], "gather", 4, "int32")  # non power of two: reference deadlocks
+    with pytest.raises(BineError):
+        plan_dump([8], "scatter", 4, "int32", root=2)
+    with pytest.raises(BineError):
+        plan_dump([4, 4], "reduce_scatter", 4, "int32", num_ports=2)  # recvbuf + offset_port quirk
+
+
+def test_fused_bytes_are_compulsory():
+    """The fused allreduce reads every input once and writes every output once."""
+    P, count = 64, 64 * 32
+    meta, items, pool, launches = plan_dump([4, 4, 4], "allreduce", count, "float32", exec="FUSED")
+    reads = sum(int(it["n"]) << int(it["lg"]) for it in items)
+    writes = sum(int(it["n"]) * int(it["nouts"]) for it in items)
+    assert reads == P * count and writes == P * count
+    assert len(launches) == 1

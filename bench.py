@@ -39,7 +39,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-buckets", action="store_true", help="skip the uneven-bucket batched vs per-step run")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--only", choices=["batched", "step", "copy"], default=None,
+                    help="profiling aid: run only that part, --steps times, with a fixed launch shape")
+    ap.add_argument("--tuning", default="2,128,0,0", help="vpt,threads,persistent,nontemporal for --only batched")
     return ap.parse_args()
 
 
@@ -84,6 +88,73 @@ def cpu_baseline(budget_s):
                       f"{reps} runs, {per:.3f} s each; GB/s = 2*P*S / time"}
 
 
+def profile_only(args, stream, send, recv, recv_step):
+    """Run one part of the benchmark for a profiler: the batched fused launch
+    with a fixed shape, the per-step launches, or a plain device copy of the
+    same 2 GiB (the reference point for HBM counters)."""
+    import torch
+    from bine_trees_fugaku_amd import Batch, VComm
+    with torch.cuda.stream(stream):
+        if args.only == "batched":
+            comm = VComm(DIMS, exec="FUSED")
+            batch = Batch(comm, [{"coll": "allreduce", "sendbuf": send, "recvbuf": recv, "count": COUNT}])
+            vpt, thr, pers, nt = (int(v) for v in args.tuning.split(","))
+            batch.set_tuning(vpt, thr, pers, nt)
+            for _ in range(args.steps):
+                batch.launch(stream)
+        elif args.only == "step":
+            comm = VComm(DIMS, exec="STEP")
+            for _ in range(args.steps):
+                comm.allreduce(send, recv_step, stream=stream)
+        else:
+            for _ in range(args.steps):
+                recv.copy_(send)
+    torch.cuda.synchronize()
+    print(json.dumps({"only": args.only, "steps": args.steps}), flush=True)
+    return 0
+
+
+def bucketed_p64(stream, seed=7, nb=256):
+    """Batched vs per-step on uneven buckets: P=64 on 4x4x4, 256 fp32 buckets of
+    log-uniform size 4 KiB..8 MiB per rank (rounded to 4 bytes), N(0,1)."""
+    import numpy as np
+    import torch
+    from bine_trees_fugaku_amd import Batch, VComm
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(4096), np.log(8 << 20), nb)).astype(np.int64) // 4  # fp32 elements
+    P64 = 64
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    sends = [torch.randn((P64, int(n)), dtype=torch.float32, device="cuda", generator=g) for n in sizes]
+    recvs = [torch.empty_like(s) for s in sends]
+    recvs2 = [torch.empty_like(s) for s in sends]
+    b_alg = int(2 * P64 * sizes.sum() * 4)
+    with torch.cuda.stream(stream):
+        comm = VComm([4, 4, 4], exec="FUSED")
+        batch = Batch(comm, [{"coll": "allreduce", "sendbuf": s, "recvbuf": r, "count": int(n)}
+                             for s, r, n in zip(sends, recvs, sizes)])
+        batch.autotune(iters=2, stream=stream)
+        b_wall, _ = time_launches(lambda: batch.launch_graph(stream), stream, 5, 2)
+        step = VComm([4, 4, 4], exec="STEP")
+
+        def per_step():
+            for s, r in zip(sends, recvs2):
+                step.allreduce(s, r, stream=stream)
+        s_wall, _ = time_launches(per_step, stream, 3, 1)
+    torch.cuda.synchronize()
+    same = all(torch.equal(a.view(torch.int32), c.view(torch.int32)) for a, c in zip(recvs, recvs2))
+    out = {"workload": "bucketed_allreduce_p64_4x4x4_fp32_256_buckets", "buckets": nb,
+           "bytes_per_rank": int(sizes.sum() * 4), "algorithmic_bytes": b_alg,
+           "batched_ms": round(b_wall, 3), "per_step_ms": round(s_wall, 3),
+           "batched_GBps": round(b_alg / (b_wall * 1e-3) / 1e9, 1), "per_step_GBps": round(b_alg / (s_wall * 1e-3) / 1e9, 1),
+           "batched_vs_per_step": round(s_wall / b_wall, 2), "per_step_launches": 13 * nb,
+           "tuning": batch.tuning(), "bitwise_equal": same}
+    batch.close()
+    comm.close()
+    step.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -102,6 +173,9 @@ def main():
     recv_step = torch.empty_like(send)
     b_alg = 2 * P * COUNT * ELEM
 
+    if args.only:
+        return profile_only(args, stream, send, recv, recv_step)
+
     with torch.cuda.stream(stream):
         fused = VComm(DIMS, exec="FUSED")
         batch = Batch(fused, [{"coll": "allreduce", "sendbuf": send, "recvbuf": recv, "count": COUNT}])
@@ -115,6 +189,11 @@ def main():
         sstats = step.plan_stats("allreduce", COUNT, "float64", exec="STEP")
         s_wall, s_ev = time_launches(lambda: step.allreduce(send, recv_step, stream=stream), stream,
                                      max(3, args.steps // 4), 2)
+        # reference point: a plain device copy of the same 2 GiB (reads S*P, writes S*P)
+        c_wall, c_ev = time_launches(lambda: recv_step.copy_(send), stream, 5, 2)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        step.allreduce(send, recv_step, stream=stream)
     torch.cuda.synchronize()
     same = bool(torch.equal(recv.view(torch.int64), recv_step.view(torch.int64)))
 
@@ -139,11 +218,16 @@ def main():
             "per_step_launches": sstats["launches"], "per_step_schedule_bytes": sstats["bytes_read"] + sstats["bytes_written"],
             "batched_vs_per_step": round(s_wall / wall_ms, 3), "pct_hbm_peak": round(100 * gbps / HBM_PEAK_GBPS, 2),
             "batched_equals_per_step_bitwise": same,
+            "device_copy_GBps": round(b_alg / (c_ev * 1e-3) / 1e9, 2),
         },
         "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": traffic},
         "cpu_baseline": None,
     }
+    del recv_step
+    torch.cuda.empty_cache()
+    if not args.no_buckets:
+        line["config"]["bucketed"] = bucketed_p64(stream)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(line), flush=True)

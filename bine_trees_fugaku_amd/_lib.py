@@ -100,8 +100,8 @@ def _sig(L):
     L.bine_batch_launch_graph.argtypes = [vp, vp]
     L.bine_batch_autotune.argtypes = [vp, vp, i32]
     L.bine_batch_stats.argtypes = [vp, ctypes.POINTER(PlanStats)]
-    L.bine_batch_get_tuning.argtypes = [vp, pint, pint, pint]
-    L.bine_batch_set_tuning.argtypes = [vp, i32, i32, i32]
+    L.bine_batch_get_tuning.argtypes = [vp, pint, pint, pint, pint]
+    L.bine_batch_set_tuning.argtypes = [vp, i32, i32, i32, i32]
     L.bine_batch_destroy.argtypes = [vp]
     L.bine_last_error.restype = ctypes.c_char_p
     L.bine_last_error.argtypes = []

@@ -131,16 +131,27 @@ class VComm:
             raise ValueError(f"buffer of {n} elements does not split into {self.size} ranks")
         return n // self.size
 
+    @staticmethod
+    def _expect(t, n: int, what: str, like=None):
+        if not t.is_cuda or not t.is_contiguous():
+            raise ValueError(f"{what} must be a contiguous GPU tensor")
+        if like is not None and t.dtype != like.dtype:
+            raise ValueError(f"{what} has element type {t.dtype}, expected {like.dtype}")
+        if t.numel() != n:
+            raise ValueError(f"{what} holds {t.numel()} elements, expected {n}")
+
     # ---- MPI entry points (lib/libbine.cc) --------------------------------------
     def allreduce(self, sendbuf, recvbuf, op: str = "SUM", stream=None):
         """MPI_Allreduce (lib/libbine.cc:1012)."""
         count = self._per_rank(sendbuf)
+        self._expect(recvbuf, self.size * count, "recvbuf", sendbuf)
         _lib.check(self._lib.bine_allreduce(sendbuf.data_ptr(), recvbuf.data_ptr(), count, _torch_dtype_code(sendbuf),
                                             OP[op], self.handle, self._stream(stream)), "bine_allreduce")
 
     def reduce_scatter(self, sendbuf, recvbuf, op: str = "SUM", stream=None):
         """MPI_Reduce_scatter with equal recvcounts (lib/libbine.cc:1064)."""
         rc = self._per_rank(recvbuf)
+        self._expect(sendbuf, self.size * self.size * rc, "sendbuf", recvbuf)
         counts = (ctypes.c_int * self.size)(*([rc] * self.size))
         _lib.check(self._lib.bine_reduce_scatter(sendbuf.data_ptr(), recvbuf.data_ptr(), counts,
                                                  _torch_dtype_code(sendbuf), OP[op], self.handle, self._stream(stream)),
@@ -149,6 +160,7 @@ class VComm:
     def allgather(self, sendbuf, recvbuf, stream=None):
         """MPI_Allgather (lib/libbine.cc:1168)."""
         c = self._per_rank(sendbuf)
+        self._expect(recvbuf, self.size * self.size * c, "recvbuf", sendbuf)
         dt = _torch_dtype_code(sendbuf)
         _lib.check(self._lib.bine_allgather(sendbuf.data_ptr(), c, dt, recvbuf.data_ptr(), c, dt, self.handle,
                                             self._stream(stream)), "bine_allgather")
@@ -162,6 +174,8 @@ class VComm:
     def alltoall(self, sendbuf, recvbuf, stream=None):
         """MPI_Alltoall (lib/libbine.cc:1291); count = elements per rank pair."""
         c = self._per_rank(sendbuf) // self.size
+        self._expect(sendbuf, self.size * self.size * c, "sendbuf")
+        self._expect(recvbuf, self.size * self.size * c, "recvbuf", sendbuf)
         dt = _torch_dtype_code(sendbuf)
         _lib.check(self._lib.bine_alltoall(sendbuf.data_ptr(), c, dt, recvbuf.data_ptr(), c, dt, self.handle,
                                            self._stream(stream)), "bine_alltoall")
@@ -169,6 +183,7 @@ class VComm:
     def scatter(self, sendbuf, recvbuf, root: int = 0, stream=None):
         """MPI_Scatter (lib/libbine.cc:1312); sendbuf is the root's P*count elements."""
         c = self._per_rank(recvbuf)
+        self._expect(sendbuf, self.size * c, "sendbuf", recvbuf)
         dt = _torch_dtype_code(recvbuf)
         _lib.check(self._lib.bine_scatter(sendbuf.data_ptr(), c, dt, recvbuf.data_ptr(), c, dt, root, self.handle,
                                           self._stream(stream)), "bine_scatter")
@@ -176,6 +191,7 @@ class VComm:
     def gather(self, sendbuf, recvbuf, root: int = 0, stream=None):
         """MPI_Gather (lib/libbine.cc:1376); recvbuf is the root's P*count elements."""
         c = self._per_rank(sendbuf)
+        self._expect(recvbuf, self.size * c, "recvbuf", sendbuf)
         dt = _torch_dtype_code(sendbuf)
         _lib.check(self._lib.bine_gather(sendbuf.data_ptr(), c, dt, recvbuf.data_ptr(), c, dt, root, self.handle,
                                          self._stream(stream)), "bine_gather")
@@ -183,6 +199,7 @@ class VComm:
     def reduce(self, sendbuf, recvbuf, op: str = "SUM", root: int = 0, stream=None):
         """MPI_Reduce (lib/libbine.cc:1440); recvbuf is the root's count elements."""
         c = self._per_rank(sendbuf)
+        self._expect(recvbuf, c, "recvbuf", sendbuf)
         _lib.check(self._lib.bine_reduce(sendbuf.data_ptr(), recvbuf.data_ptr(), c, _torch_dtype_code(sendbuf),
                                          OP[op], root, self.handle, self._stream(stream)), "bine_reduce")
 
@@ -277,14 +294,15 @@ class Batch:
     def autotune(self, iters: int = 3, stream=None):
         _lib.check(self._lib.bine_batch_autotune(self.handle, VComm._stream(stream), iters), "bine_batch_autotune")
 
-    def set_tuning(self, vpt: int, threads: int, persistent: int = 0):
-        _lib.check(self._lib.bine_batch_set_tuning(self.handle, vpt, threads, persistent), "bine_batch_set_tuning")
+    def set_tuning(self, vpt: int, threads: int, persistent: int = 0, nontemporal: int = 0):
+        _lib.check(self._lib.bine_batch_set_tuning(self.handle, vpt, threads, persistent, nontemporal),
+                   "bine_batch_set_tuning")
 
     def tuning(self) -> dict:
-        v, t, p = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _lib.check(self._lib.bine_batch_get_tuning(self.handle, ctypes.byref(v), ctypes.byref(t), ctypes.byref(p)),
-                   "bine_batch_get_tuning")
-        return {"vpt": v.value, "threads": t.value, "persistent_wgs": p.value}
+        v, t, p, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(self._lib.bine_batch_get_tuning(self.handle, ctypes.byref(v), ctypes.byref(t), ctypes.byref(p),
+                                                   ctypes.byref(n)), "bine_batch_get_tuning")
+        return {"vpt": v.value, "threads": t.value, "persistent_wgs": p.value, "nontemporal": n.value}
 
     def stats(self) -> dict:
         st = _lib.PlanStats()

@@ -78,11 +78,13 @@ int max_lg_of(const std::vector<std::vector<Group>>& gs) {
     return m;
 }
 
-int upload(const std::vector<std::vector<Group>>& gs, size_t ds, int64_t tile_elems, uint32_t bucket, DevArrays* d) {
+int upload(const std::vector<std::vector<Group>>& gs, size_t ds, const LaunchShape& sh, uint32_t bucket, DevArrays* d) {
     d->release();
-    lower_groups(gs, tile_elems, (int)(16 / ds), bucket, ds, &d->low);
+    lower_groups(gs, shallow_tile_elems(sh, ds), (int)(16 / ds), bucket, ds, &d->low, deep_tile_elems(sh, ds),
+                 split_lg(sh));
     d->max_lg = max_lg_of(gs);
-    d->tile_elems = tile_elems;
+    d->tile_elems = shallow_tile_elems(sh, ds);
+    d->shape = sh;
     if (!d->low.items.empty()) {
         HIPCHK(hipMalloc(&d->items, d->low.items.size() * sizeof(GredItem)));
         HIPCHK(hipMemcpy(d->items, d->low.items.data(), d->low.items.size() * sizeof(GredItem), hipMemcpyHostToDevice));
@@ -123,7 +125,9 @@ struct PlanKeyHash {
     }
 };
 
-int64_t default_tile_elems(size_t ds) { return (int64_t)(4096 / ds); }
+// Default launch shape: 256 threads, one 16-byte vector each (4 KiB tiles).
+LaunchShape default_shape() { return LaunchShape{1, 256, 0, 0}; }
+int64_t default_tile_elems(size_t ds) { return shallow_tile_elems(default_shape(), ds); }
 
 }  // namespace
 
@@ -157,8 +161,7 @@ int get_plan(bine_vcomm_t comm, const CollRequest& req, int dtype, DevPlan** out
     if (!compile_program(prog, true, &dp->cp, &err)) return fail(BINE_ERR_UNSUPPORTED, err);
     dp->dtype = dtype;
     dp->ds = ds;
-    const int64_t tile = default_tile_elems(ds);
-    int rc = upload(dp->cp.step_groups, ds, tile, 0, &dp->step);
+    int rc = upload(dp->cp.step_groups, ds, default_shape(), 0, &dp->step);
     if (rc) return rc;
     if (dp->cp.fused_ok && max_lg_of(std::vector<std::vector<Group>>(1, dp->cp.fused)) > 14) {
         dp->cp.fused_ok = false;  // the kernel's register stack holds trees of up to 2^14 leaves
@@ -167,7 +170,7 @@ int get_plan(bine_vcomm_t comm, const CollRequest& req, int dtype, DevPlan** out
     }
     if (dp->cp.fused_ok) {
         std::vector<std::vector<Group>> one(1, dp->cp.fused);
-        rc = upload(one, ds, tile, 0, &dp->fused);
+        rc = upload(one, ds, default_shape(), 0, &dp->fused);
         if (rc) return rc;
     }
     // temporaries are allocated only when some step touches them
@@ -498,6 +501,15 @@ extern "C" int bine_plan_stats(bine_vcomm_t comm, int32_t coll, int count, bine_
 
 namespace {
 
+// Launch shapes the autotuners time: vectors per thread, workgroup size,
+// persistent grid, streaming loads/stores.
+std::vector<LaunchShape> tuning_candidates(int cus) {
+    return {{1, 64, 0, 0},  {1, 128, 0, 0}, {1, 256, 0, 0}, {1, 512, 0, 0}, {2, 64, 0, 0},
+            {2, 128, 0, 0}, {2, 256, 0, 0}, {2, 512, 0, 0}, {4, 64, 0, 0},  {4, 128, 0, 0},
+            {1, 256, 0, 1}, {2, 256, 0, 1}, {1, 512, 0, 1}, {2, 128, 0, 1}, {1, 256, 4 * cus, 0},
+            {2, 128, 8 * cus, 0}};
+}
+
 // Issue one call of a collective (argument layout of its entry point).
 int issue(bine_vcomm_t comm, int32_t coll, const void* send, void* recv, int count, int dtype, int op, int root,
           void* stream) {
@@ -542,25 +554,16 @@ extern "C" int bine_plan_autotune(bine_vcomm_t comm, int32_t coll, const void* s
     int cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, comm->cfg.device) == hipSuccess) cus = prop.multiProcessorCount;
-    struct Cand {
-        int vpt, threads, persistent;
-    };
-    const Cand cands[] = {{1, 64, 0}, {1, 128, 0}, {1, 256, 0}, {2, 64, 0}, {2, 128, 0}, {2, 256, 0}, {4, 64, 0},
-                          {4, 128, 0}, {4, 256, 0}, {1, 512, 0}, {1, 256, 4 * cus}, {2, 256, 4 * cus}, {1, 64, 16 * cus}};
+    const std::vector<LaunchShape> cands = tuning_candidates(cus);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     hipStream_t st = (hipStream_t)stream;
     float best = 1e30f;
-    Cand bestc = {d.shape.vpt, d.shape.threads, d.shape.persistent};
-    int64_t best_tile = d.tile_elems;
+    LaunchShape bestc = d.shape;
     rc = BINE_OK;
-    for (const Cand& c : cands) {
-        const int64_t tile = (int64_t)c.threads * c.vpt * (int64_t)(16 / dp->ds);
-        if (tile != d.tile_elems && (rc = upload(groups, dp->ds, tile, 0, &d))) break;
-        d.shape.vpt = c.vpt;
-        d.shape.threads = c.threads;
-        d.shape.persistent = c.persistent;
+    for (const LaunchShape& c : cands) {
+        if ((rc = upload(groups, dp->ds, c, 0, &d))) break;
         if ((rc = issue(comm, coll, sendbuf, recvbuf, count, datatype, op, root, stream))) break;
         if (hipEventRecord(e0, st) != hipSuccess) { rc = fail(BINE_ERR_HIP, "event"); break; }
         for (int i = 0; i < iters && !rc; ++i) rc = issue(comm, coll, sendbuf, recvbuf, count, datatype, op, root, stream);
@@ -572,18 +575,13 @@ extern "C" int bine_plan_autotune(bine_vcomm_t comm, int32_t coll, const void* s
         if (ms < best) {
             best = ms;
             bestc = c;
-            best_tile = tile;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     comm->cfg.exec = saved_exec;
     if (rc) return rc;
-    if (best_tile != d.tile_elems && (rc = upload(groups, dp->ds, best_tile, 0, &d))) return rc;
-    d.shape.vpt = bestc.vpt;
-    d.shape.threads = bestc.threads;
-    d.shape.persistent = bestc.persistent;
-    return BINE_OK;
+    return upload(groups, dp->ds, bestc, 0, &d);
 }
 
 extern "C" int bine_plan_dump(const bine_vcomm_config_t* cfg, int32_t coll, int count, bine_datatype_t datatype,
@@ -603,10 +601,14 @@ extern "C" int bine_plan_dump(const bine_vcomm_config_t* cfg, int32_t coll, int 
     if (!compile_program(prog, true, &cp, &err)) return fail(BINE_ERR_UNSUPPORTED, err);
     if (exec == BINE_EXEC_AUTO) exec = cp.fused_ok ? BINE_EXEC_FUSED : BINE_EXEC_STEP;
     if (exec == BINE_EXEC_FUSED && !cp.fused_ok) return fail(BINE_ERR_UNSUPPORTED, "schedule cannot be fused: " + cp.fused_why);
+    LaunchShape sh = default_shape();
+    int64_t deep = deep_tile_elems(sh, ds);
     if (tile_elems <= 0) tile_elems = default_tile_elems(ds);
+    else deep = tile_elems;
     Lowered low;
-    if (exec == BINE_EXEC_FUSED) lower_groups(std::vector<std::vector<Group>>(1, cp.fused), tile_elems, (int)(16 / ds), 0, ds, &low);
-    else lower_groups(cp.step_groups, tile_elems, (int)(16 / ds), 0, ds, &low);
+    if (exec == BINE_EXEC_FUSED)
+        lower_groups(std::vector<std::vector<Group>>(1, cp.fused), tile_elems, (int)(16 / ds), 0, ds, &low, deep, split_lg(sh));
+    else lower_groups(cp.step_groups, tile_elems, (int)(16 / ds), 0, ds, &low, deep, split_lg(sh));
     info->n_items = low.items.size();
     info->n_pool = low.pool.size();
     info->n_launches = low.launches.size();
@@ -713,8 +715,7 @@ struct bine_batch_s {
     std::vector<TypeList> lists;
     BufTable* d_tabs = nullptr;
     std::vector<BufTable> tabs;
-    LaunchShape shape;
-    int64_t tile_bytes = 4096;
+    LaunchShape shape = default_shape();
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipStream_t graph_stream = nullptr;
@@ -738,8 +739,8 @@ int batch_lower(bine_batch_t b) {
         for (size_t gi = 0; gi < l.groups[0].size(); ++gi) {
             Lowered one;
             std::vector<std::vector<Group>> v(1, std::vector<Group>{l.groups[0][gi]});
-            lower_groups(v, std::max<int64_t>(16 / (int64_t)ds, b->tile_bytes / (int64_t)ds), (int)(16 / ds),
-                         l.group_bucket[gi], ds, &one);
+            lower_groups(v, shallow_tile_elems(b->shape, ds), (int)(16 / ds), l.group_bucket[gi], ds, &one,
+                         deep_tile_elems(b->shape, ds), split_lg(b->shape));
             const uint32_t base = (uint32_t)all.pool.size();
             for (auto it : one.items) {
                 it.leaf_off += base;
@@ -885,61 +886,52 @@ extern "C" int bine_batch_stats(bine_batch_t b, bine_plan_stats_t* out) {
     return BINE_OK;
 }
 
-extern "C" int bine_batch_get_tuning(bine_batch_t b, int32_t* vpt, int32_t* threads, int32_t* persistent_wgs) {
+extern "C" int bine_batch_get_tuning(bine_batch_t b, int32_t* vpt, int32_t* threads, int32_t* persistent_wgs,
+                                     int32_t* nontemporal) {
     if (!b) return fail(BINE_ERR_ARG, "NULL batch");
     if (vpt) *vpt = b->shape.vpt;
     if (threads) *threads = b->shape.threads;
     if (persistent_wgs) *persistent_wgs = b->shape.persistent;
+    if (nontemporal) *nontemporal = b->shape.nontemporal;
     return BINE_OK;
 }
 
-extern "C" int bine_batch_set_tuning(bine_batch_t b, int32_t vpt, int32_t threads, int32_t persistent_wgs) {
+namespace {
+void drop_graph(bine_batch_t b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+}  // namespace
+
+extern "C" int bine_batch_set_tuning(bine_batch_t b, int32_t vpt, int32_t threads, int32_t persistent_wgs,
+                                     int32_t nontemporal) {
     if (!b) return fail(BINE_ERR_ARG, "NULL batch");
-    if ((vpt != 1 && vpt != 2 && vpt != 4) || threads < 64 || threads > 512 || (threads & 63) || persistent_wgs < 0)
+    if ((vpt != 1 && vpt != 2 && vpt != 4) || threads < 64 || threads > 512 || (threads & (threads - 1)) ||
+        persistent_wgs < 0)
         return fail(BINE_ERR_ARG, "bad launch shape");
-    b->shape.vpt = vpt;
-    b->shape.threads = threads;
-    b->shape.persistent = persistent_wgs;
-    const int64_t tb = (int64_t)threads * vpt * 16;
-    if (tb != b->tile_bytes) {
-        b->tile_bytes = tb;
-        if (b->gexec) {
-            (void)hipGraphExecDestroy(b->gexec);
-            (void)hipGraphDestroy(b->graph);
-            b->gexec = nullptr;
-            b->graph = nullptr;
-        }
-        return batch_lower(b);
-    }
-    if (b->gexec) {
-        (void)hipGraphExecDestroy(b->gexec);
-        (void)hipGraphDestroy(b->graph);
-        b->gexec = nullptr;
-        b->graph = nullptr;
-    }
-    return BINE_OK;
+    LaunchShape sh{vpt, threads, persistent_wgs, nontemporal ? 1 : 0};
+    const bool relower = sh.vpt != b->shape.vpt || sh.threads != b->shape.threads;
+    b->shape = sh;
+    drop_graph(b);
+    return relower ? batch_lower(b) : BINE_OK;
 }
 
 extern "C" int bine_batch_autotune(bine_batch_t b, void* stream, int iters) {
     if (!b || iters < 1) return fail(BINE_ERR_ARG, "bad arguments");
     HIPCHK(hipSetDevice(b->comm->cfg.device));
     hipStream_t st = (hipStream_t)stream;
-    struct Cand {
-        int vpt, threads, persistent;
-    };
     int cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, b->comm->cfg.device) == hipSuccess) cus = prop.multiProcessorCount;
-    std::vector<Cand> cands = {{1, 64, 0},   {1, 128, 0},  {1, 256, 0}, {2, 128, 0},
-                               {2, 256, 0},  {4, 256, 0},  {1, 512, 0}, {2, 512, 0},
-                               {1, 256, 4 * cus}, {2, 256, 4 * cus}, {1, 128, 8 * cus}, {4, 256, 2 * cus}};
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     float best = 1e30f;
-    Cand bestc = {b->shape.vpt, b->shape.threads, b->shape.persistent};
-    for (const Cand& c : cands) {
-        int rc = bine_batch_set_tuning(b, c.vpt, c.threads, c.persistent);
+    LaunchShape bestc = b->shape;
+    for (const LaunchShape& c : tuning_candidates(cus)) {
+        int rc = bine_batch_set_tuning(b, c.vpt, c.threads, c.persistent, c.nontemporal);
         if (rc) return rc;
         rc = batch_run(b, st);  // warm-up
         if (rc) return rc;
@@ -959,7 +951,7 @@ extern "C" int bine_batch_autotune(bine_batch_t b, void* stream, int iters) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    return bine_batch_set_tuning(b, bestc.vpt, bestc.threads, bestc.persistent);
+    return bine_batch_set_tuning(b, bestc.vpt, bestc.threads, bestc.persistent, bestc.nontemporal);
 }
 
 extern "C" int bine_batch_destroy(bine_batch_t b) {

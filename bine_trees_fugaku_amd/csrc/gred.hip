@@ -129,6 +129,12 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
                                                    const PoolEntry* __restrict__ pool, BufTable tab,
                                                    const BufTable* __restrict__ tabs, uint32_t allow_vec) {
     constexpr int VE = Vec<T>::N;
+    // partial subtree sums of the waves, [wave][u][lane] 16-byte vectors
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    Vec<T>* lds = reinterpret_cast<Vec<T>*>(lds_raw);
+    const uint32_t nw = blockDim.x >> 6;
+    const uint32_t lgw = __builtin_ctz(nw);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     for (uint32_t ii = blockIdx.x; ii < nitems; ii += gridDim.x) {
         const GredItem it = items[ii];
         Table t;
@@ -142,10 +148,69 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
         const PoolEntry* __restrict__ outs = pool + it.out_off;
         const uint32_t lg = it.lg;
         const uint64_t delta = it.delta;
-        if ((it.flags & ITEM_VEC) && allow_vec) {
+        auto addp = [](const Pack<T, VPT>& a, const Pack<T, VPT>& b) __attribute__((always_inline)) {
+            Pack<T, VPT> r;
+#pragma unroll
+            for (int u = 0; u < VPT; ++u) r.x[u] = vadd<T>(a.x[u], b.x[u]);
+            return r;
+        };
+        const bool vec = (it.flags & ITEM_VEC) && allow_vec;
+        if (vec && nw > 1 && lg >= lgw + 3) {
+            // Deep tree: wave w reduces the contiguous leaf range [w << lgs, (w+1) << lgs),
+            // a whole subtree of the balanced tree; the top lgw levels are then
+            // joined pairwise from LDS, so the association is unchanged.
+            const uint32_t lgs = lg - lgw;
+            const PoolEntry* __restrict__ my = leaves + ((size_t)wave << lgs);
+            const uint32_t nvec = it.n / VE;
+            for (uint32_t base = 0; base < nvec; base += 64 * VPT) {
+                uint32_t vi[VPT];
+                bool ok[VPT];
+#pragma unroll
+                for (int u = 0; u < VPT; ++u) {
+                    vi[u] = base + lane + u * 64;
+                    ok[u] = vi[u] < nvec;
+                }
+                auto loadp = [&](uint32_t k) __attribute__((always_inline)) {
+                    const PoolEntry e = my[k];
+                    const char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
+                    Pack<T, VPT> r;
+#pragma unroll
+                    for (int u = 0; u < VPT; ++u) {
+                        if (ok[u]) r.x[u] = vload<T, NT>(p + (uint64_t)vi[u] * 16);
+                        else r.x[u] = Vec<T>{};
+                    }
+                    return r;
+                };
+                Pack<T, VPT> part = tree_eval<Pack<T, VPT>, MAXS>(lgs, addp, loadp);
+#pragma unroll
+                for (int u = 0; u < VPT; ++u) lds[((size_t)wave * VPT + u) * 64 + lane] = part.x[u];
+                __syncthreads();
+                // join the wave subtrees pairwise in place: level st adds wave w+st into w
+                for (uint32_t st = 1; st < nw; st <<= 1) {
+                    if ((wave & (2 * st - 1)) == 0) {
+#pragma unroll
+                        for (int u = 0; u < VPT; ++u) {
+                            Vec<T>& a = lds[((size_t)wave * VPT + u) * 64 + lane];
+                            a = vadd<T>(a, lds[((size_t)(wave + st) * VPT + u) * 64 + lane]);
+                        }
+                    }
+                    __syncthreads();
+                }
+                Pack<T, VPT> sum;
+#pragma unroll
+                for (int u = 0; u < VPT; ++u) sum.x[u] = lds[(size_t)u * 64 + lane];
+                __syncthreads();  // the next pass reuses the LDS slots
+                for (uint32_t o = wave; o < it.nouts; o += nw) {
+                    const PoolEntry e = outs[o];
+                    char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
+#pragma unroll
+                    for (int u = 0; u < VPT; ++u)
+                        if (ok[u]) vstore<T, NT>(p + (uint64_t)vi[u] * 16, sum.x[u]);
+                }
+            }
+        } else if (vec) {
             const uint32_t nvec = it.n / VE;
             for (uint32_t base = 0; base < nvec; base += blockDim.x * VPT) {
-                // lane-local vector indices of this pass
                 uint32_t vi[VPT];
                 bool ok[VPT];
 #pragma unroll
@@ -153,12 +218,6 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
                     vi[u] = base + threadIdx.x + u * blockDim.x;
                     ok[u] = vi[u] < nvec;
                 }
-                auto addp = [](const Pack<T, VPT>& a, const Pack<T, VPT>& b) __attribute__((always_inline)) {
-                    Pack<T, VPT> r;
-#pragma unroll
-                    for (int u = 0; u < VPT; ++u) r.x[u] = vadd<T>(a.x[u], b.x[u]);
-                    return r;
-                };
                 auto loadp = [&](uint32_t k) __attribute__((always_inline)) {
                     const PoolEntry e = leaves[k];
                     const char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
@@ -210,11 +269,13 @@ static hipError_t launch_t(const GredItem* items, uint32_t n, const PoolEntry* p
     uint32_t grid = sh.persistent > 0 ? (uint32_t)sh.persistent : n;
     if (grid > n) grid = n;
     if (grid == 0) return hipSuccess;
+    if (sh.threads < 64 || sh.threads > 512 || (sh.threads & (sh.threads - 1))) return hipErrorInvalidValue;
+    const size_t lds = (size_t)(sh.threads / 64) * VPT * 64 * 16;
     if (sh.nontemporal)
-        hipLaunchKernelGGL((gred_kernel<T, VPT, MAXS, BATCH, true>), dim3(grid), dim3(sh.threads), 0, st, items, n,
+        hipLaunchKernelGGL((gred_kernel<T, VPT, MAXS, BATCH, true>), dim3(grid), dim3(sh.threads), lds, st, items, n,
                            pool, tab, tabs, allow_vec);
     else
-        hipLaunchKernelGGL((gred_kernel<T, VPT, MAXS, BATCH, false>), dim3(grid), dim3(sh.threads), 0, st, items, n,
+        hipLaunchKernelGGL((gred_kernel<T, VPT, MAXS, BATCH, false>), dim3(grid), dim3(sh.threads), lds, st, items, n,
                            pool, tab, tabs, allow_vec);
     return hipGetLastError();
 }

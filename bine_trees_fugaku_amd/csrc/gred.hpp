@@ -39,9 +39,16 @@ struct BufTable {
 // Launch shape of the gather-reduce kernel.
 struct LaunchShape {
     int vpt = 1;          // 16-byte vectors per thread per pass: 1, 2 or 4
-    int threads = 256;    // workgroup size: 64..512
+    int threads = 256;    // workgroup size: 64..512 (a multiple of 64)
     int persistent = 0;   // 0: one workgroup per item; >0: grid of that many workgroups
     int nontemporal = 0;  // streaming loads/stores
 };
+
+// Items whose tree has at least 2^(log2(waves) + 3) leaves are reduced with
+// their leaves split across the waves of the workgroup; their tile is one
+// wave wide.  Shallower items spread their elements over all threads.
+inline int split_lg(const LaunchShape& s) { return __builtin_ctz((unsigned)(s.threads / 64)) + 3; }
+inline int64_t shallow_tile_elems(const LaunchShape& s, size_t ds) { return (int64_t)s.threads * s.vpt * (int64_t)(16 / ds); }
+inline int64_t deep_tile_elems(const LaunchShape& s, size_t ds) { return (int64_t)64 * s.vpt * (int64_t)(16 / ds); }
 
 }  // namespace bine

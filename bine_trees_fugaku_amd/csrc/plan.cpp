@@ -1647,12 +1647,13 @@ bool compile_program(const Program& prog, bool want_fused, CompiledPlan* out, st
 // Lowering
 // ===========================================================================
 
-void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t tile_elems, int vec, uint32_t bucket,
-                  size_t dtsize, Lowered* out) {
+void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shallow_tile, int vec, uint32_t bucket,
+                  size_t dtsize, Lowered* out, int64_t deep_tile_elems, int deep_lg) {
     for (const auto& gs : launches) {
         const uint32_t first = (uint32_t)out->items.size();
         for (const Group& g : gs) {
             if (g.n <= 0) continue;
+            const int64_t tile_elems = (g.lg >= deep_lg && deep_tile_elems > 0) ? deep_tile_elems : shallow_tile;
             const uint32_t leaf_off = (uint32_t)out->pool.size();
             out->pool.insert(out->pool.end(), g.leaves.begin(), g.leaves.end());
             const uint32_t out_off = (uint32_t)out->pool.size();

@@ -109,7 +109,10 @@ struct Lowered {
     uint64_t bytes_read = 0, bytes_written = 0, max_leaves = 0;
 };
 // tile_elems: elements per work item; vec_elems: elements per 16-byte vector.
+// Groups whose tree has at least 2^deep_lg leaves use deep_tile_elems instead:
+// the kernel splits their leaves across the waves of a workgroup, so their
+// tile is one wave wide.
 void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t tile_elems, int vec_elems,
-                  uint32_t bucket, size_t dtsize, Lowered* out);
+                  uint32_t bucket, size_t dtsize, Lowered* out, int64_t deep_tile_elems = 0, int deep_lg = 99);
 
 }  // namespace bine

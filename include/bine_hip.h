@@ -219,9 +219,12 @@ int bine_batch_launch_graph(bine_batch_t batch, void* stream);
 /* Time candidate (vector width, workgroup, persistence) shapes on the batch and keep the fastest. */
 int bine_batch_autotune(bine_batch_t batch, void* stream, int iters);
 int bine_batch_stats(bine_batch_t batch, bine_plan_stats_t* out);
-/* Chosen launch shape: vectors per thread, threads per workgroup, persistent grid (0 = one workgroup per item). */
-int bine_batch_get_tuning(bine_batch_t batch, int32_t* vpt, int32_t* threads, int32_t* persistent_wgs);
-int bine_batch_set_tuning(bine_batch_t batch, int32_t vpt, int32_t threads, int32_t persistent_wgs);
+/* Chosen launch shape: 16-byte vectors per thread (1, 2, 4), threads per workgroup (64..512, power of
+ * two), persistent grid (0 = one workgroup per item), streaming (non-temporal) loads and stores. */
+int bine_batch_get_tuning(bine_batch_t batch, int32_t* vpt, int32_t* threads, int32_t* persistent_wgs,
+                          int32_t* nontemporal);
+int bine_batch_set_tuning(bine_batch_t batch, int32_t vpt, int32_t threads, int32_t persistent_wgs,
+                          int32_t nontemporal);
 int bine_batch_destroy(bine_batch_t batch);
 
 /* Last error message of the calling thread (static storage). */

@@ -23,7 +23,8 @@ EXPORTED = [
     "bine_compute_step_info", "bine_block_step", "bine_get_tree", "bine_remap_rank", "bine_reduce_local",
     "bine_plan_stats", "bine_plan_autotune", "bine_plan_dump", "bine_batch_create", "bine_batch_launch", "bine_batch_launch_graph",
     "bine_batch_autotune", "bine_batch_stats", "bine_batch_get_tuning", "bine_batch_set_tuning",
-    "bine_batch_destroy", "bine_last_error",
+    "bine_batch_destroy", "bine_sim_classes", "bine_sim_peers", "bine_sim_create", "bine_sim_run", "bine_sim_case_info",
+    "bine_sim_class_stats", "bine_sim_link_counts", "bine_sim_destroy", "bine_last_error",
 ]
 
 
@@ -54,6 +55,11 @@ class PlanDumpInfo(ctypes.Structure):
     _fields_ = [("n_items", ctypes.c_uint64), ("n_pool", ctypes.c_uint64), ("n_launches", ctypes.c_uint64),
                 ("extent", ctypes.c_int64 * 4), ("single", ctypes.c_int32 * 4), ("P", ctypes.c_uint32),
                 ("fused_ok", ctypes.c_int32)]
+
+
+class SimCase(ctypes.Structure):
+    _fields_ = [("dimensions", ctypes.c_uint32 * 3), ("dimensions_num", ctypes.c_uint32), ("coll", ctypes.c_int32),
+                ("algo", ctypes.c_int32)]
 
 
 class BatchItem(ctypes.Structure):
@@ -103,6 +109,16 @@ def _sig(L):
     L.bine_batch_get_tuning.argtypes = [vp, pint, pint, pint, pint]
     L.bine_batch_set_tuning.argtypes = [vp, i32, i32, i32, i32]
     L.bine_batch_destroy.argtypes = [vp]
+    psc = ctypes.POINTER(SimCase)
+    pu64 = ctypes.POINTER(ctypes.c_uint64)
+    L.bine_sim_classes.argtypes = [psc, pu32, pu32, pu32, pu32, pu32, pu32, pu32]
+    L.bine_sim_peers.argtypes = [psc, u32, pu32]
+    L.bine_sim_create.argtypes = [psc, i32, i32, ctypes.POINTER(vp)]
+    L.bine_sim_run.argtypes = [vp, vp]
+    L.bine_sim_case_info.argtypes = [vp, i32, pu32, pu32]
+    L.bine_sim_class_stats.argtypes = [vp, i32, pu32, pu32, pu32, pu32, pu64, pu32]
+    L.bine_sim_link_counts.argtypes = [vp, i32, u32, pu32]
+    L.bine_sim_destroy.argtypes = [vp]
     L.bine_last_error.restype = ctypes.c_char_p
     L.bine_last_error.argtypes = []
 

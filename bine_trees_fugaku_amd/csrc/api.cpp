@@ -19,6 +19,7 @@
 #include "bine_hip.h"
 #include "gred.hpp"
 #include "plan.hpp"
+#include "status.hpp"
 
 namespace bine {
 hipError_t launch_gred(int dtype, int max_lg, const GredItem* items, uint32_t n, const PoolEntry* pool,
@@ -130,6 +131,12 @@ LaunchShape default_shape() { return LaunchShape{1, 256, 0, 0}; }
 int64_t default_tile_elems(size_t ds) { return shallow_tile_elems(default_shape(), ds); }
 
 }  // namespace
+
+// Error text shared with the other C-ABI translation units (sim, selector).
+int bine::record_error(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
 
 struct bine_vcomm_s {
     bine_vcomm_config_t cfg;
@@ -504,10 +511,11 @@ namespace {
 // Launch shapes the autotuners time: vectors per thread, workgroup size,
 // persistent grid, streaming loads/stores.
 std::vector<LaunchShape> tuning_candidates(int cus) {
-    return {{1, 64, 0, 0},  {1, 128, 0, 0}, {1, 256, 0, 0}, {1, 512, 0, 0}, {2, 64, 0, 0},
-            {2, 128, 0, 0}, {2, 256, 0, 0}, {2, 512, 0, 0}, {4, 64, 0, 0},  {4, 128, 0, 0},
-            {1, 256, 0, 1}, {2, 256, 0, 1}, {1, 512, 0, 1}, {2, 128, 0, 1}, {1, 256, 4 * cus, 0},
-            {2, 128, 8 * cus, 0}};
+    return {{1, 64, 0, 0},  {1, 128, 0, 0}, {1, 256, 0, 0}, {1, 512, 0, 0},       {2, 64, 0, 0},
+            {2, 128, 0, 0}, {2, 256, 0, 0}, {2, 512, 0, 0}, {4, 64, 0, 0},        {4, 128, 0, 0},
+            {1, 256, 0, 1}, {2, 256, 0, 1}, {1, 512, 0, 1}, {2, 128, 0, 1},       {2, 64, 0, 1},
+            {4, 64, 0, 1},  {4, 128, 0, 1}, {1, 64, 0, 1},  {4, 64, 8 * cus, 0},  {4, 64, 16 * cus, 1},
+            {1, 256, 4 * cus, 0}, {2, 128, 8 * cus, 0}};
 }
 
 // Issue one call of a collective (argument layout of its entry point).

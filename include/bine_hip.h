@@ -227,6 +227,58 @@ int bine_batch_set_tuning(bine_batch_t batch, int32_t vpt, int32_t threads, int3
                           int32_t nontemporal);
 int bine_batch_destroy(bine_batch_t batch);
 
+/* ---- torus link-traffic simulation --------------------------------------
+ * Replaces simulation/simulator.py (run_collectives :125-177, get_peer :70-76)
+ * and extends it with per-link accounting, which the reference lacks: every
+ * message of a schedule step is routed over the torus (dimension-ordered,
+ * shorter way round, positive way on a tie) and the GPU counts, per directed
+ * link, how many messages cross it.  Ranks sit on the torus in the library's
+ * row-major order (lib/libbine_common.cc:206-246), except for the
+ * simulator.py algorithm, whose own id order (simulator.py:11-26) is kept.
+ *
+ * A case is (torus, collective, algorithm).  Its schedule is a list of step
+ * classes; every message of a class carries the share num/den of the per-rank
+ * vector and the class repeats rep times, so the bytes a link carries in a
+ * class at S bytes per rank are count * floor(S * num / den), an exact integer. */
+typedef enum {
+    BINE_SIM_BINE = 0,    /* Bine schedules (torus peers for allreduce/RS/AG, 1D trees and butterflies otherwise) */
+    BINE_SIM_RECDOUB = 1, /* recursive doubling / binomial trees */
+    BINE_SIM_RING = 2,    /* ring reduce-scatter / allgather / allreduce */
+    BINE_SIM_BRUCK = 3,   /* Bruck alltoall (lib/libbine_coll_alltoall.cc:18-107) */
+    BINE_SIM_SIMPY = 4    /* simulation/simulator.py's own peers and id order (equal dimensions) */
+} bine_sim_algo_t;
+
+typedef struct {
+    uint32_t dimensions[3];
+    uint32_t dimensions_num;
+    int32_t coll; /* bine_coll_t */
+    int32_t algo; /* bine_sim_algo_t */
+} bine_sim_case_t;
+
+typedef struct bine_sim_s* bine_sim_t;
+
+/* Step classes of one case (host only).  Arrays may be NULL; otherwise they
+ * hold at least *nclasses entries (call once with NULL arrays to size them;
+ * a case has at most 2 * 40 classes). */
+int bine_sim_classes(const bine_sim_case_t* c, uint32_t* nclasses, uint32_t* kind, uint32_t* d, uint32_t* k,
+                     uint32_t* num, uint32_t* den, uint32_t* rep);
+/* Destination of every rank's message in class cls (host only; P entries,
+ * UINT32_MAX for a rank that sends nothing).  For BINE_SIM_SIMPY, ranks and
+ * destinations are simulator.py ids. */
+int bine_sim_peers(const bine_sim_case_t* c, uint32_t cls, uint32_t* dest);
+/* Upload n cases and allocate their link counters on `device`. */
+int bine_sim_create(const bine_sim_case_t* cases, int n, int device, bine_sim_t* out);
+/* Count every class of every case: one GPU pass of integer atomics. */
+int bine_sim_run(bine_sim_t sim, void* stream);
+int bine_sim_case_info(bine_sim_t sim, int i, uint32_t* nclasses, uint32_t* nlinks);
+/* Per class of case i (synchronous copies): share, repetitions, the most
+ * loaded link's message count, the sum of link crossings, the longest route. */
+int bine_sim_class_stats(bine_sim_t sim, int i, uint32_t* num, uint32_t* den, uint32_t* rep, uint32_t* max_mult,
+                         uint64_t* sum_mult, uint32_t* max_hops);
+/* Message count of every directed link (nlinks entries) in class cls of case i. */
+int bine_sim_link_counts(bine_sim_t sim, int i, uint32_t cls, uint32_t* counts);
+int bine_sim_destroy(bine_sim_t sim);
+
 /* Last error message of the calling thread (static storage). */
 const char* bine_last_error(void);
 

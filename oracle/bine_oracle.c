@@ -553,6 +553,69 @@ static int msg_key_cmp(const void* a, const void* b) {
     return i < j ? -1 : (i > j);
 }
 
+/* message trace (or_trace_*) */
+static int g_trace_on = 0;
+static int32_t g_trace_step = 0;
+static size_t g_trace_n = 0, g_trace_cap = 0;
+static int32_t* g_trace_i32 = NULL; /* step, src, dst triples */
+static uint64_t* g_trace_bytes = NULL;
+
+void or_trace_start(void) {
+    g_trace_on = 1;
+    g_trace_step = 0;
+    g_trace_n = 0;
+}
+size_t or_trace_stop(void) {
+    g_trace_on = 0;
+    return g_trace_n;
+}
+void or_trace_fetch(int32_t* step, int32_t* src, int32_t* dst, uint64_t* bytes) {
+    for (size_t i = 0; i < g_trace_n; i++) {
+        step[i] = g_trace_i32[3 * i];
+        src[i] = g_trace_i32[3 * i + 1];
+        dst[i] = g_trace_i32[3 * i + 2];
+        bytes[i] = g_trace_bytes[i];
+    }
+}
+static void trace_msg(int src, int dst, size_t bytes) {
+    if (g_trace_n == g_trace_cap) {
+        g_trace_cap = g_trace_cap ? 2 * g_trace_cap : 4096;
+        g_trace_i32 = (int32_t*)realloc(g_trace_i32, sizeof(int32_t) * 3 * g_trace_cap);
+        g_trace_bytes = (uint64_t*)realloc(g_trace_bytes, sizeof(uint64_t) * g_trace_cap);
+    }
+    g_trace_i32[3 * g_trace_n] = g_trace_step;
+    g_trace_i32[3 * g_trace_n + 1] = src;
+    g_trace_i32[3 * g_trace_n + 2] = dst;
+    g_trace_bytes[g_trace_n++] = bytes;
+}
+
+uint32_t or_route_bytes(const uint32_t* dims, uint32_t ndims, size_t n, const int32_t* src, const int32_t* dst,
+                        const uint64_t* bytes, uint64_t* link_bytes) {
+    or_torus_t t;
+    if (or_torus_init(&t, dims, ndims) != OR_OK) return 0;
+    uint32_t longest = 0;
+    for (size_t i = 0; i < n; i++) {
+        int a[OR_MAX_DIMS], b[OR_MAX_DIMS];
+        or_coord_from_id(&t, src[i], a);
+        or_coord_from_id(&t, dst[i], b);
+        uint32_t hops = 0;
+        for (uint32_t d = 0; d < ndims; d++) {
+            int dim = (int)dims[d];
+            int fwd = or_mod(b[d] - a[d], dim); /* hops going the positive way */
+            int neg = 2 * fwd > dim;
+            int h = neg ? dim - fwd : fwd;
+            for (int k = 0; k < h; k++) {
+                size_t node = (size_t)or_id_from_coord(&t, a);
+                link_bytes[(node * ndims + d) * 2 + (size_t)neg] += bytes[i];
+                a[d] = or_mod(a[d] + (neg ? -1 : 1), dim);
+            }
+            hops += (uint32_t)h;
+        }
+        if (hops > longest) longest = hops;
+    }
+    return longest;
+}
+
 static int step_run(step_t* s) {
     size_t total = 0;
     for (size_t i = 0; i < s->ns; i++) total += s->sends[i].bytes;
@@ -581,7 +644,9 @@ static int step_run(step_t* s) {
             break;
         }
         memcpy((void*)r->ptr, stage + soff[si[k]], snd->bytes);
+        if (g_trace_on && snd->src != snd->dst) trace_msg(snd->src, snd->dst, snd->bytes);
     }
+    if (g_trace_on) g_trace_step++;
     free(si);
     free(ri);
     free(soff);

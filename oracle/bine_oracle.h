@@ -115,10 +115,20 @@ int or_sim_peer(int sender, int step, const uint32_t* dims, uint32_t nd, int all
 int or_sim_run(const uint32_t* dims, uint32_t nd, int collective, const int* distances,
                double* data, double* out_stats);
 
-/* ---- link-traffic model (extension; defined in DESIGN) ----------------- */
-/* Per directed link byte counts for one collective: link id = (node*ndims + d)*2 + dir. */
-int or_link_traffic(const uint32_t* dims, uint32_t ndims, int family, int coll, uint64_t msg_bytes,
-                    uint64_t* link_bytes, uint64_t* step_max_link, uint64_t* total_byte_hops);
+/* ---- message trace and link routing (extension: the reference has no
+ * per-link accounting) -------------------------------------------------- */
+/* While tracing is on, every message the step engine delivers is recorded as
+ * (engine step, src, dst, bytes); the step counter restarts at or_trace_start. */
+void or_trace_start(void);
+size_t or_trace_stop(void); /* returns the number of recorded messages */
+void or_trace_fetch(int32_t* step, int32_t* src, int32_t* dst, uint64_t* bytes);
+/* Dimension-ordered minimal routing on the torus (row-major ids, last dimension
+ * fastest): dimension 0 first; on a dimension the shorter way round, the
+ * positive way on a tie.  Adds bytes[i] to every directed link message i
+ * crosses; link id = (node*ndims + d)*2 + (negative ? 1 : 0).  Returns the hops
+ * of the longest route. */
+uint32_t or_route_bytes(const uint32_t* dims, uint32_t ndims, size_t n, const int32_t* src, const int32_t* dst,
+                        const uint64_t* bytes, uint64_t* link_bytes);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,7 @@ OP = {"SUM": 0, "VOID": 1}
 NP = {"int32": np.int32, "int64": np.int64, "float32": np.float32, "float64": np.float64, "int8": np.int8}
 
 _lock = threading.Lock()
+_trace_lock = threading.Lock()  # one traced collective at a time (the trace is process-global)
 _lib = None
 
 
@@ -94,6 +95,12 @@ def lib():
             L.or_reverse.restype = u32
             L.or_sim_peer.argtypes = [i32, i32, pu32, u32, i32, ctypes.POINTER(ctypes.c_int)]
             L.or_sim_run.argtypes = [pu32, u32, i32, ctypes.POINTER(ctypes.c_int), vp, vp]
+            L.or_trace_start.argtypes = []
+            L.or_trace_stop.argtypes = []
+            L.or_trace_stop.restype = ctypes.c_size_t
+            L.or_trace_fetch.argtypes = [vp, vp, vp, vp]
+            L.or_route_bytes.argtypes = [pu32, u32, ctypes.c_size_t, vp, vp, vp, vp]
+            L.or_route_bytes.restype = u32
             _lib = L
     return _lib
 
@@ -300,3 +307,38 @@ def sim_run(dims, data: np.ndarray, collective="ALLREDUCE", distances=JACOBSTHAL
     code = {"ALLREDUCE": 0, "REDUCESCATTER": 1, "ALLGATHER": 2}[collective]
     _check(lib().or_sim_run(_u32(dims), len(dims), code, d, _ptr(data), _ptr(stats)), "sim_run")
     return data, stats
+
+
+# ---- message trace and link routing -------------------------------------------
+
+def trace(fn, *args, **kwargs):
+    """Run one oracle collective with the engine's message trace on.
+
+    Returns (result, msgs): msgs holds int32 arrays step/src/dst and uint64
+    bytes, one entry per delivered message between distinct ranks, in the
+    order the engine delivered them."""
+    L = lib()
+    with _trace_lock:
+        L.or_trace_start()
+        try:
+            res = fn(*args, **kwargs)
+        finally:
+            n = L.or_trace_stop()
+        msgs = {k: np.zeros(n, np.int32) for k in ("step", "src", "dst")}
+        msgs["bytes"] = np.zeros(n, np.uint64)
+        if n:
+            L.or_trace_fetch(_ptr(msgs["step"]), _ptr(msgs["src"]), _ptr(msgs["dst"]), _ptr(msgs["bytes"]))
+    return res, msgs
+
+
+def route_bytes(dims, src, dst, nbytes):
+    """Dimension-ordered minimal routing of messages src[i] -> dst[i] carrying
+    nbytes[i]; returns (bytes per directed link, longest route in hops).
+    Link id = (node * ndims + d) * 2 + (1 if the hop goes the negative way)."""
+    src = np.ascontiguousarray(src, np.int32)
+    dst = np.ascontiguousarray(dst, np.int32)
+    nbytes = np.ascontiguousarray(np.broadcast_to(np.asarray(nbytes, np.uint64), src.shape))
+    P = int(np.prod(dims))
+    out = np.zeros(P * len(dims) * 2, np.uint64)
+    longest = lib().or_route_bytes(_u32(dims), len(dims), len(src), _ptr(src), _ptr(dst), _ptr(nbytes), _ptr(out))
+    return out, int(longest)

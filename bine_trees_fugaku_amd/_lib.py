@@ -24,7 +24,9 @@ EXPORTED = [
     "bine_plan_stats", "bine_plan_autotune", "bine_plan_dump", "bine_batch_create", "bine_batch_launch", "bine_batch_launch_graph",
     "bine_batch_autotune", "bine_batch_stats", "bine_batch_get_tuning", "bine_batch_set_tuning",
     "bine_batch_destroy", "bine_sim_classes", "bine_sim_peers", "bine_sim_create", "bine_sim_run", "bine_sim_case_info",
-    "bine_sim_class_stats", "bine_sim_link_counts", "bine_sim_destroy", "bine_last_error",
+    "bine_sim_class_stats", "bine_sim_link_counts", "bine_sim_destroy", "bine_mlp_create", "bine_mlp_num_params",
+    "bine_mlp_get_params", "bine_mlp_set_params", "bine_mlp_forward", "bine_mlp_gradients", "bine_mlp_train_step",
+    "bine_mlp_destroy", "bine_last_error",
 ]
 
 
@@ -119,6 +121,15 @@ def _sig(L):
     L.bine_sim_class_stats.argtypes = [vp, i32, pu32, pu32, pu32, pu32, pu64, pu32]
     L.bine_sim_link_counts.argtypes = [vp, i32, u32, pu32]
     L.bine_sim_destroy.argtypes = [vp]
+    pf = ctypes.POINTER(ctypes.c_float)
+    L.bine_mlp_create.argtypes = [ctypes.POINTER(ctypes.c_int32), i32, i32, ctypes.c_uint64, i32, ctypes.POINTER(vp)]
+    L.bine_mlp_num_params.argtypes = [vp, pu64]
+    L.bine_mlp_get_params.argtypes = [vp, pf]
+    L.bine_mlp_set_params.argtypes = [vp, pf]
+    L.bine_mlp_forward.argtypes = [vp, vp, i32, vp, vp]
+    L.bine_mlp_gradients.argtypes = [vp, vp, vp, vp, i32, vp, vp, vp]
+    L.bine_mlp_train_step.argtypes = [vp, vp, vp, vp, i32, ctypes.c_float, vp, vp]
+    L.bine_mlp_destroy.argtypes = [vp]
     L.bine_last_error.restype = ctypes.c_char_p
     L.bine_last_error.argtypes = []
 

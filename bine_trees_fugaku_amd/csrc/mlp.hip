@@ -1,0 +1,416 @@
+// mlp.hip -- fp32 multilayer perceptron (ReLU hidden layers) for the
+// algorithm selector: forward, backward and Adam on the GPU.
+//
+// Layer l maps A_{l-1} (n x in_l) to Z_l = A_{l-1} W_l^T + b_l (n x out_l);
+// hidden layers apply ReLU, the last layer is linear.  Parameters live in one
+// flat fp32 buffer, layer by layer: W_l (out_l x in_l, row-major), then b_l.
+//
+// Every product is one tiled GEMM on the matrix cores: a 64x64 output tile
+// per 256-thread workgroup, each wave owning a 32x32 quarter built from 2x2
+// v_mfma_f32_16x16x4f32 blocks, operands staged through LDS in K chunks of 16.
+// The loss is a masked mean squared error; every reduction runs in a fixed
+// order, so gradients are deterministic run to run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "bine_hip.h"
+#include "status.hpp"
+
+namespace bine {
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int TM = 64, TN = 64, TK = 16;
+
+enum Epi : int {
+    EPI_BIAS = 0,       // C = acc + bias[col]
+    EPI_BIAS_RELU = 1,  // C = max(acc + bias[col], 0)
+    EPI_RELU_MASK = 2,  // C = acc * (mask[row, col] > 0)   (dZ of a ReLU layer)
+    EPI_PLAIN = 3,      // C = acc
+};
+
+// C[M x N] = op(A)[M x K] * op(B)[K x N] with the epilogue above.
+// op(A)[m][k] = TA ? A[k * lda + m] : A[m * lda + k]
+// op(B)[k][n] = TB ? B[n * ldb + k] : B[k * ldb + n]
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_f32(int M, int N, int K, const float* __restrict__ A, int lda,
+                                                const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
+                                                const float* __restrict__ bias, const float* __restrict__ mask, int epi) {
+    __shared__ float As[TK][TM + 4];  // As[k][m]
+    __shared__ float Bs[TK][TN + 4];  // Bs[k][n]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    floatx4 acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    for (int k0 = 0; k0 < K; k0 += TK) {
+        // 64 x 16 elements of each operand, 4 per thread.
+        for (int e = tid; e < TM * TK; e += 256) {
+            int mm, kk;
+            if (TA) {  // consecutive threads walk m (contiguous in memory)
+                mm = e % TM;
+                kk = e / TM;
+            } else {  // consecutive threads walk k
+                kk = e % TK;
+                mm = e / TK;
+            }
+            const int gm = m0 + mm, gk = k0 + kk;
+            float v = 0.f;
+            if (gm < M && gk < K) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
+            As[kk][mm] = v;
+        }
+        for (int e = tid; e < TN * TK; e += 256) {
+            int nn, kk;
+            if (TB) {
+                kk = e % TK;
+                nn = e / TK;
+            } else {
+                nn = e % TN;
+                kk = e / TN;
+            }
+            const int gn = n0 + nn, gk = k0 + kk;
+            float v = 0.f;
+            if (gn < N && gk < K) v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
+            Bs[kk][nn] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < TK; ks += 4) {
+            // 16x16x4: lane supplies A[row = lane % 16][k = lane / 16] and B[k = lane / 16][col = lane % 16]
+            const int kk = ks + (lane >> 4);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const float a = As[kk][wm + i * 16 + (lane & 15)];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const float b = Bs[kk][wn + j * 16 + (lane & 15)];
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i][j], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // Output layout of a 16x16 block: lane holds rows 4 * (lane / 16) + r, column lane % 16.
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+                const int col = n0 + wn + j * 16 + (lane & 15);
+                if (row >= M || col >= N) continue;
+                float v = acc[i][j][r];
+                if (epi == EPI_BIAS || epi == EPI_BIAS_RELU) v += bias[col];
+                if (epi == EPI_BIAS_RELU) v = v > 0.f ? v : 0.f;
+                if (epi == EPI_RELU_MASK) v = mask[(size_t)row * ldc + col] > 0.f ? v : 0.f;
+                C[(size_t)row * ldc + col] = v;
+            }
+}
+
+// dY = 2 * mask * (Y - T) / max(count, 1); per-block partial sums of the loss
+// terms mask * (Y - T)^2 and of the mask, reduced in a fixed order.
+__global__ __launch_bounds__(256) void mse_grad(int n_elems, const float* __restrict__ Y, const float* __restrict__ T,
+                                                const float* __restrict__ Mk, float* __restrict__ dY,
+                                                float* __restrict__ partial, int pass, const float* __restrict__ totals) {
+    __shared__ float s_l[256], s_c[256];
+    float l = 0.f, c = 0.f;
+    const float inv = pass ? 2.f / fmaxf(totals[1], 1.f) : 0.f;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n_elems; i += gridDim.x * 256) {
+        const float m = Mk[i];
+        const float d = m > 0.f ? Y[i] - T[i] : 0.f;
+        if (pass) dY[i] = inv * d;
+        l += d * d;
+        c += m > 0.f ? 1.f : 0.f;
+    }
+    s_l[threadIdx.x] = l;
+    s_c[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            s_l[threadIdx.x] += s_l[threadIdx.x + s];
+            s_c[threadIdx.x] += s_c[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && !pass) {
+        partial[2 * blockIdx.x] = s_l[0];
+        partial[2 * blockIdx.x + 1] = s_c[0];
+    }
+}
+
+// totals = {sum of loss terms, mask count}; loss = totals[0] / max(totals[1], 1).
+__global__ void reduce_partials(int nb, const float* __restrict__ partial, float* __restrict__ totals,
+                                float* __restrict__ loss_out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float l = 0.f, c = 0.f;
+    for (int i = 0; i < nb; ++i) {
+        l += partial[2 * i];
+        c += partial[2 * i + 1];
+    }
+    totals[0] = l;
+    totals[1] = c;
+    if (loss_out) loss_out[0] = l / fmaxf(c, 1.f);
+}
+
+// db[j] = sum_i dZ[i][j]: one workgroup per 64 columns, rows split over 4 waves, fixed order.
+__global__ __launch_bounds__(256) void col_sum(int n, int cols, const float* __restrict__ dZ, float* __restrict__ db) {
+    __shared__ float s[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = blockIdx.x * 64 + lane;
+    float acc = 0.f;
+    if (col < cols)
+        for (int i = wave; i < n; i += 4) acc += dZ[(size_t)i * cols + col];
+    s[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && col < cols) db[col] = (s[0][lane] + s[1][lane]) + (s[2][lane] + s[3][lane]);
+}
+
+__global__ void adam_update(uint64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+template <bool TA, bool TB>
+void gemm(int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C, int ldc, const float* bias,
+          const float* mask, int epi, hipStream_t st) {
+    dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
+    hipLaunchKernelGGL((gemm_f32<TA, TB>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, bias, mask, epi);
+}
+
+constexpr int kLossBlocks = 120;
+
+}  // namespace
+}  // namespace bine
+
+using namespace bine;
+
+struct bine_mlp_s {
+    int device = 0;
+    std::vector<int> dims;  // in, hidden..., out
+    int max_batch = 0;
+    uint64_t nparams = 0;
+    std::vector<uint64_t> w_off, b_off;
+    float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    std::vector<float*> act;  // act[l]: n x dims[l+1] outputs of layer l (post-ReLU for hidden layers)
+    float *dz0 = nullptr, *dz1 = nullptr;  // ping-pong n x max width
+    float *partial = nullptr, *totals = nullptr;
+    uint64_t step = 0;
+    ~bine_mlp_s() {
+        for (float* p : {params, grads, adam_m, adam_v, dz0, dz1, partial, totals})
+            if (p) (void)hipFree(p);
+        for (float* p : act)
+            if (p) (void)hipFree(p);
+    }
+    int layers() const { return (int)dims.size() - 1; }
+};
+
+namespace {
+
+int mfail(int code, const std::string& m) { return record_error(code, m); }
+
+#define MHIP(x)                                                                                           \
+    do {                                                                                                  \
+        hipError_t e_ = (x);                                                                              \
+        if (e_ != hipSuccess) return mfail(BINE_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Forward pass over n rows of x; leaves every layer's output in m->act.
+void forward(bine_mlp_t m, const float* x, int n, hipStream_t st) {
+    const float* in = x;
+    for (int l = 0; l < m->layers(); ++l) {
+        const int fi = m->dims[l], fo = m->dims[l + 1];
+        const bool last = l == m->layers() - 1;
+        gemm<false, true>(n, fo, fi, in, fi, m->params + m->w_off[l], fi, m->act[l], fo, m->params + m->b_off[l],
+                          nullptr, last ? EPI_BIAS : EPI_BIAS_RELU, st);
+        in = m->act[l];
+    }
+}
+
+// Loss and gradients of the masked MSE into m->grads (and *loss_out when given).
+void backward(bine_mlp_t m, const float* x, const float* t, const float* mask, int n, float* loss_out, hipStream_t st) {
+    const int L = m->layers(), fo = m->dims[L];
+    const int ne = n * fo;
+    const int nb = std::min(kLossBlocks, (ne + 255) / 256);
+    float* y = m->act[L - 1];
+    hipLaunchKernelGGL(mse_grad, dim3(nb), dim3(256), 0, st, ne, y, t, mask, m->dz0, m->partial, 0, m->totals);
+    hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(1), 0, st, nb, m->partial, m->totals, loss_out);
+    hipLaunchKernelGGL(mse_grad, dim3(nb), dim3(256), 0, st, ne, y, t, mask, m->dz0, m->partial, 1, m->totals);
+    float* dz = m->dz0;
+    float* other = m->dz1;
+    for (int l = L - 1; l >= 0; --l) {
+        const int fi = m->dims[l], fout = m->dims[l + 1];
+        const float* a_prev = l ? m->act[l - 1] : x;
+        // dW_l = dZ_l^T A_{l-1}  (fout x fi), K = n
+        gemm<true, false>(fout, fi, n, dz, fout, a_prev, fi, m->grads + m->w_off[l], fi, nullptr, nullptr, EPI_PLAIN, st);
+        hipLaunchKernelGGL(col_sum, dim3((fout + 63) / 64), dim3(256), 0, st, n, fout, dz, m->grads + m->b_off[l]);
+        if (l) {  // dZ_{l-1} = (dZ_l W_l) * (A_{l-1} > 0)
+            gemm<false, false>(n, fi, fout, dz, fout, m->params + m->w_off[l], fi, other, fi, nullptr, m->act[l - 1],
+                               EPI_RELU_MASK, st);
+            std::swap(dz, other);
+        }
+    }
+}
+
+int check_batch(bine_mlp_t m, const void* x, int n) {
+    if (!m) return mfail(BINE_ERR_ARG, "NULL network");
+    if (!x) return mfail(BINE_ERR_ARG, "NULL input");
+    if (n <= 0 || n > m->max_batch) return mfail(BINE_ERR_ARG, "batch size must be in [1, max_batch]");
+    return BINE_OK;
+}
+
+}  // namespace
+
+extern "C" int bine_mlp_create(const int32_t* dims, int32_t ndims, int32_t max_batch, uint64_t seed, int device,
+                               bine_mlp_t* out) {
+    if (!dims || ndims < 2 || ndims > 16 || max_batch <= 0 || !out) return mfail(BINE_ERR_ARG, "bad arguments");
+    for (int i = 0; i < ndims; ++i)
+        if (dims[i] <= 0 || dims[i] > 4096) return mfail(BINE_ERR_ARG, "layer widths must be in [1, 4096]");
+    auto m = std::make_unique<bine_mlp_s>();
+    m->device = device;
+    m->dims.assign(dims, dims + ndims);
+    m->max_batch = max_batch;
+    uint64_t off = 0;
+    int widest = 0;
+    for (int l = 0; l + 1 < ndims; ++l) {
+        m->w_off.push_back(off);
+        off += (uint64_t)dims[l] * dims[l + 1];
+        m->b_off.push_back(off);
+        off += (uint64_t)dims[l + 1];
+    }
+    for (int i = 0; i < ndims; ++i) widest = std::max(widest, (int)dims[i]);
+    m->nparams = off;
+    // He-uniform weights from splitmix64, zero biases.
+    std::vector<float> h(off, 0.f);
+    uint64_t s = seed;
+    for (int l = 0; l + 1 < ndims; ++l) {
+        const float bound = std::sqrt(6.f / (float)dims[l]);
+        for (uint64_t i = 0; i < (uint64_t)dims[l] * dims[l + 1]; ++i) {
+            const double u = (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0);
+            h[m->w_off[l] + i] = (float)((2.0 * u - 1.0) * bound);
+        }
+    }
+    MHIP(hipSetDevice(device));
+    const size_t pb = sizeof(float) * off;
+    MHIP(hipMalloc(&m->params, pb));
+    MHIP(hipMalloc(&m->grads, pb));
+    MHIP(hipMalloc(&m->adam_m, pb));
+    MHIP(hipMalloc(&m->adam_v, pb));
+    MHIP(hipMemcpy(m->params, h.data(), pb, hipMemcpyHostToDevice));
+    MHIP(hipMemset(m->adam_m, 0, pb));
+    MHIP(hipMemset(m->adam_v, 0, pb));
+    MHIP(hipMemset(m->grads, 0, pb));
+    for (int l = 0; l + 1 < ndims; ++l) {
+        float* a = nullptr;
+        MHIP(hipMalloc(&a, sizeof(float) * (size_t)max_batch * dims[l + 1]));
+        m->act.push_back(a);
+    }
+    MHIP(hipMalloc(&m->dz0, sizeof(float) * (size_t)max_batch * widest));
+    MHIP(hipMalloc(&m->dz1, sizeof(float) * (size_t)max_batch * widest));
+    MHIP(hipMalloc(&m->partial, sizeof(float) * 2 * kLossBlocks));
+    MHIP(hipMalloc(&m->totals, sizeof(float) * 2));
+    *out = m.release();
+    return BINE_OK;
+}
+
+extern "C" int bine_mlp_num_params(bine_mlp_t m, uint64_t* n) {
+    if (!m || !n) return mfail(BINE_ERR_ARG, "NULL argument");
+    *n = m->nparams;
+    return BINE_OK;
+}
+
+extern "C" int bine_mlp_get_params(bine_mlp_t m, float* host) {
+    if (!m || !host) return mfail(BINE_ERR_ARG, "NULL argument");
+    MHIP(hipSetDevice(m->device));
+    MHIP(hipMemcpy(host, m->params, sizeof(float) * m->nparams, hipMemcpyDeviceToHost));
+    return BINE_OK;
+}
+
+extern "C" int bine_mlp_set_params(bine_mlp_t m, const float* host) {
+    if (!m || !host) return mfail(BINE_ERR_ARG, "NULL argument");
+    MHIP(hipSetDevice(m->device));
+    MHIP(hipMemcpy(m->params, host, sizeof(float) * m->nparams, hipMemcpyHostToDevice));
+    const size_t pb = sizeof(float) * m->nparams;
+    MHIP(hipMemset(m->adam_m, 0, pb));
+    MHIP(hipMemset(m->adam_v, 0, pb));
+    m->step = 0;
+    return BINE_OK;
+}
+
+extern "C" int bine_mlp_forward(bine_mlp_t m, const float* x, int32_t n, float* y, void* stream) {
+    int rc = check_batch(m, x, n);
+    if (rc) return rc;
+    if (!y) return mfail(BINE_ERR_ARG, "NULL output");
+    MHIP(hipSetDevice(m->device));
+    hipStream_t st = (hipStream_t)stream;
+    forward(m, x, n, st);
+    MHIP(hipMemcpyAsync(y, m->act.back(), sizeof(float) * (size_t)n * m->dims.back(), hipMemcpyDeviceToDevice, st));
+    MHIP(hipGetLastError());
+    return BINE_OK;
+}
+
+extern "C" int bine_mlp_gradients(bine_mlp_t m, const float* x, const float* t, const float* mask, int32_t n,
+                                  float* grads, float* loss, void* stream) {
+    int rc = check_batch(m, x, n);
+    if (rc) return rc;
+    if (!t || !mask || !grads) return mfail(BINE_ERR_ARG, "NULL target, mask or gradient buffer");
+    MHIP(hipSetDevice(m->device));
+    hipStream_t st = (hipStream_t)stream;
+    forward(m, x, n, st);
+    backward(m, x, t, mask, n, loss, st);
+    MHIP(hipMemcpyAsync(grads, m->grads, sizeof(float) * m->nparams, hipMemcpyDeviceToDevice, st));
+    MHIP(hipGetLastError());
+    return BINE_OK;
+}
+
+extern "C" int bine_mlp_train_step(bine_mlp_t m, const float* x, const float* t, const float* mask, int32_t n,
+                                   float lr, float* loss, void* stream) {
+    int rc = check_batch(m, x, n);
+    if (rc) return rc;
+    if (!t || !mask) return mfail(BINE_ERR_ARG, "NULL target or mask");
+    if (!(lr > 0.f)) return mfail(BINE_ERR_ARG, "learning rate must be positive");
+    MHIP(hipSetDevice(m->device));
+    hipStream_t st = (hipStream_t)stream;
+    forward(m, x, n, st);
+    backward(m, x, t, mask, n, loss, st);
+    m->step += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
+    const uint64_t np = m->nparams;
+    const int blocks = (int)std::min<uint64_t>(1024, (np + 255) / 256);
+    hipLaunchKernelGGL(adam_update, dim3(blocks), dim3(256), 0, st, np, m->params, m->grads, m->adam_m, m->adam_v, lr,
+                       b1, b2, 1e-8f, c1, c2);
+    MHIP(hipGetLastError());
+    return BINE_OK;
+}
+
+extern "C" int bine_mlp_destroy(bine_mlp_t m) {
+    if (!m) return BINE_OK;
+    (void)hipSetDevice(m->device);
+    delete m;
+    return BINE_OK;
+}

@@ -279,6 +279,31 @@ int bine_sim_class_stats(bine_sim_t sim, int i, uint32_t* num, uint32_t* den, ui
 int bine_sim_link_counts(bine_sim_t sim, int i, uint32_t cls, uint32_t* counts);
 int bine_sim_destroy(bine_sim_t sim);
 
+/* ---- cost-model network of the algorithm selector -------------------------
+ * No counterpart in the reference, whose selection is the fixed defaults of
+ * lib/libbine.cc:28-73 (and Fujitsu MPI's hand-written fjmpi-dectree.conf).
+ * An fp32 multilayer perceptron: dims = {inputs, hidden..., outputs}; ReLU
+ * after every hidden layer, linear output.  Inputs x (n x inputs), targets t
+ * and masks (n x outputs) are row-major fp32 device buffers owned by the
+ * caller; the loss is sum(mask * (y - t)^2) / count(mask > 0).  Training uses
+ * Adam (beta 0.9 / 0.999, eps 1e-8).  Calls are stream-ordered. */
+typedef struct bine_mlp_s* bine_mlp_t;
+
+/* He-uniform weights from splitmix64(seed), zero biases; buffers for batches up to max_batch rows. */
+int bine_mlp_create(const int32_t* dims, int32_t ndims, int32_t max_batch, uint64_t seed, int device, bine_mlp_t* out);
+int bine_mlp_num_params(bine_mlp_t mlp, uint64_t* n);
+/* Flat parameters, layer by layer: W (outputs x inputs, row-major) then b. Synchronous. */
+int bine_mlp_get_params(bine_mlp_t mlp, float* host);
+int bine_mlp_set_params(bine_mlp_t mlp, const float* host); /* also resets the Adam state */
+int bine_mlp_forward(bine_mlp_t mlp, const float* x, int32_t n, float* y, void* stream);
+/* Loss (one float, device) and its gradient w.r.t. every parameter (device, num_params floats); no update. */
+int bine_mlp_gradients(bine_mlp_t mlp, const float* x, const float* t, const float* mask, int32_t n, float* grads,
+                       float* loss, void* stream);
+/* One Adam step on the batch; loss (device float, may be NULL) is the loss before the step. */
+int bine_mlp_train_step(bine_mlp_t mlp, const float* x, const float* t, const float* mask, int32_t n, float lr,
+                        float* loss, void* stream);
+int bine_mlp_destroy(bine_mlp_t mlp);
+
 /* Last error message of the calling thread (static storage). */
 const char* bine_last_error(void);
 

@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-buckets", action="store_true", help="skip the uneven-bucket batched vs per-step run")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-collective batch run")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--only", choices=["batched", "step", "copy"], default=None,
                     help="profiling aid: run only that part, --steps times, with a fixed launch shape")
@@ -155,6 +156,67 @@ def bucketed_p64(stream, seed=7, nb=256):
     return out
 
 
+def mixed_8x8x8(stream, seed=21):
+    """Mixed collective batch on 8x8x8 (P=512): allreduce fp64 4 MiB per rank,
+    alltoall int64 4 KiB per rank pair, bcast int32 64 KiB, scatter fp32 256 KiB
+    blocks, gather int32 1 MiB blocks, reduce fp32 1 MiB; seeded uniform
+    integers in [-1000, 1000] and N(0,1).  Batched: one fused work list per
+    element type, captured in one HIP graph.  Baseline: the six collectives as
+    six single calls (each one fused launch; per-step execution of the rooted
+    collectives at these sizes needs the reference's full-size temporary on
+    every rank, P x 512 MiB for the gather)."""
+    import torch
+    from bine_trees_fugaku_amd import Batch, VComm
+    Pm = 512
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+
+    def ri(shape, dt):
+        return torch.randint(-1000, 1001, shape, dtype=dt, device="cuda", generator=g)
+
+    def rn(shape, dt):
+        return torch.randn(shape, dtype=dt, device="cuda", generator=g)
+    c_ar, c_a2a, c_b, c_s, c_g, c_r = 524288, 512, 16384, 65536, 262144, 262144
+    t = {
+        "allreduce": (rn((Pm, c_ar), torch.float64), torch.empty((Pm, c_ar), dtype=torch.float64, device="cuda"), c_ar),
+        "alltoall": (ri((Pm, Pm * c_a2a), torch.int64), torch.empty((Pm, Pm * c_a2a), dtype=torch.int64, device="cuda"), c_a2a),
+        "bcast": (None, ri((Pm, c_b), torch.int32), c_b),
+        "scatter": (rn((Pm * c_s,), torch.float32), torch.empty((Pm, c_s), dtype=torch.float32, device="cuda"), c_s),
+        "gather": (ri((Pm, c_g), torch.int32), torch.empty((Pm * c_g,), dtype=torch.int32, device="cuda"), c_g),
+        "reduce": (rn((Pm, c_r), torch.float32), torch.empty((c_r,), dtype=torch.float32, device="cuda"), c_r),
+    }
+    # compulsory bytes per collective: read every input once, write every output once
+    b_alg = (2 * Pm * c_ar * 8 + 2 * Pm * Pm * c_a2a * 8 + (Pm + 1) * c_b * 4 + 2 * Pm * c_s * 4 + 2 * Pm * c_g * 4
+             + (Pm + 1) * c_r * 4)
+    with torch.cuda.stream(stream):
+        comm = VComm(DIMS, exec="FUSED")
+        batch = Batch(comm, [{"coll": c, "sendbuf": s, "recvbuf": r, "count": n} for c, (s, r, n) in t.items()])
+        batch.autotune(iters=2, stream=stream)
+        b_wall, _ = time_launches(lambda: batch.launch_graph(stream), stream, 10, 3)
+        outs = {c: r.clone() for c, (s, r, n) in t.items() if c != "bcast"}
+        step = VComm(DIMS, exec="AUTO")
+
+        def singles():
+            step.allreduce(t["allreduce"][0], t["allreduce"][1], stream=stream)
+            step.alltoall(t["alltoall"][0], t["alltoall"][1], stream=stream)
+            step.bcast(t["bcast"][1], stream=stream)
+            step.scatter(t["scatter"][0], t["scatter"][1], stream=stream)
+            step.gather(t["gather"][0], t["gather"][1], stream=stream)
+            step.reduce(t["reduce"][0], t["reduce"][1], stream=stream)
+        s_wall, _ = time_launches(singles, stream, 3, 1)
+    torch.cuda.synchronize()
+    same = all(torch.equal(outs[c].view(torch.uint8), t[c][1].view(torch.uint8)) for c in outs)
+    out = {"workload": "mixed_8x8x8_allreduce_fp64_4MiB_alltoall_int64_4KiB_bcast_scatter_gather_reduce",
+           "algorithmic_bytes": int(b_alg), "batched_ms": round(b_wall, 3), "per_call_ms": round(s_wall, 3),
+           "batched_GBps": round(b_alg / (b_wall * 1e-3) / 1e9, 1), "per_call_GBps": round(b_alg / (s_wall * 1e-3) / 1e9, 1),
+           "batched_vs_per_call": round(s_wall / b_wall, 2), "stats": batch.stats(), "tuning": batch.tuning(),
+           "bitwise_equal": same}
+    batch.close()
+    comm.close()
+    step.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -228,6 +290,10 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_buckets:
         line["config"]["bucketed"] = bucketed_p64(stream)
+        torch.cuda.empty_cache()
+    if not args.no_mixed:
+        line["config"]["mixed_8x8x8"] = mixed_8x8x8(stream)
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(line), flush=True)

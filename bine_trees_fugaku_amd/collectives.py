@@ -257,6 +257,24 @@ def plan_dump(dims: Sequence[int], coll: str, count: int, dtype, op: str = "SUM"
     return meta, items, pool, launches
 
 
+def _batch_sizes(coll: str, P: int, count: int):
+    """(sendbuf elements or None, recvbuf elements) of one batch item, laid out
+    as for the single call with that MPI count argument."""
+    sizes = {
+        "allreduce": (P * count, P * count),
+        "reduce_scatter": (P * P * count, P * count),
+        "allgather": (P * count, P * P * count),
+        "bcast": (None, P * count),
+        "alltoall": (P * P * count, P * P * count),
+        "scatter": (P * count, P * count),
+        "gather": (P * count, P * count),
+        "reduce": (P * count, count),
+    }
+    if coll not in sizes:
+        raise ValueError(f"unknown collective {coll!r}")
+    return sizes[coll]
+
+
 class Batch:
     """Many collectives on one vcomm compiled into one fused work list.
 
@@ -273,6 +291,12 @@ class Batch:
         for i, it in enumerate(items):
             recv = it["recvbuf"]
             send = it.get("sendbuf")
+            n_send, n_recv = _batch_sizes(it["coll"], comm.size, int(it["count"]))
+            VComm._expect(recv, n_recv, f"batch item {i} recvbuf")
+            if n_send is not None:
+                if send is None:
+                    raise ValueError(f"batch item {i} ({it['coll']}) needs a sendbuf")
+                VComm._expect(send, n_send, f"batch item {i} sendbuf", recv)
             self._keep.append((send, recv))
             arr[i].coll = COLL[it["coll"]]
             arr[i].sendbuf = send.data_ptr() if send is not None else None

@@ -102,7 +102,8 @@ struct DevPlan {
     int dtype = 0;
     size_t ds = 0;
     DevArrays step, fused;
-    void* tmp[2] = {nullptr, nullptr};  // BUF_TMP, BUF_TMP2 workspaces
+    void* tmp[2] = {nullptr, nullptr};  // BUF_TMP, BUF_TMP2 workspaces, allocated on the first per-step run
+    size_t tmp_bytes[2] = {0, 0};         // 0: the per-step schedule never touches that buffer
     ~DevPlan() {
         step.release();
         fused.release();
@@ -187,14 +188,29 @@ int get_plan(bine_vcomm_t comm, const CollRequest& req, int dtype, DevPlan** out
             for (const PoolEntry& e : g.leaves) used[e.rb >> 28] = true;
             for (const PoolEntry& e : g.outs) used[e.rb >> 28] = true;
         }
-    for (int b = BUF_TMP; b <= BUF_TMP2; ++b) {
-        if (prog.extent[b] > 0 && used[b]) {
-            size_t bytes = (size_t)prog.extent[b] * ds * (prog.single[b] ? 1 : prog.P);
-            HIPCHK(hipMalloc(&dp->tmp[b - BUF_TMP], bytes));
-        }
-    }
+    for (int b = BUF_TMP; b <= BUF_TMP2; ++b)
+        if (prog.extent[b] > 0 && used[b]) dp->tmp_bytes[b - BUF_TMP] = (size_t)prog.extent[b] * ds * (prog.single[b] ? 1 : prog.P);
     *out = dp.get();
     comm->plans.emplace(key, std::move(dp));
+    return BINE_OK;
+}
+
+// Allocate the per-step schedule's temporaries (the fused schedule needs none).
+// Every rank keeps the reference's full-size temporary, so per-step execution
+// of large rooted collectives can need P times the reference's per-process memory.
+int ensure_tmp(bine_vcomm_t comm, DevPlan* dp) {
+    std::lock_guard<std::mutex> lock(comm->mu);
+    for (int i = 0; i < 2; ++i) {
+        if (!dp->tmp_bytes[i] || dp->tmp[i]) continue;
+        HIPCHK(hipSetDevice(comm->cfg.device));
+        hipError_t e = hipMalloc(&dp->tmp[i], dp->tmp_bytes[i]);
+        if (e != hipSuccess) {
+            dp->tmp[i] = nullptr;
+            (void)hipGetLastError();
+            return fail(BINE_ERR_HIP, "per-step execution needs " + std::to_string(dp->tmp_bytes[i] >> 20) +
+                                          " MiB of temporaries: " + hipGetErrorString(e));
+        }
+    }
     return BINE_OK;
 }
 
@@ -271,6 +287,7 @@ int run_coll(bine_vcomm_t comm, const CollRequest& req, int dtype, const void* s
     if (exec == BINE_EXEC_FUSED && !dp->cp.fused_ok)
         return fail(BINE_ERR_UNSUPPORTED, "schedule cannot be fused: " + dp->cp.fused_why);
     if (exec == BINE_EXEC_AUTO) exec = dp->cp.fused_ok ? BINE_EXEC_FUSED : BINE_EXEC_STEP;
+    if (exec == BINE_EXEC_STEP && (rc = ensure_tmp(comm, dp))) return rc;
     HIPCHK(hipSetDevice(comm->cfg.device));
     BufTable tab = make_table(dp, send, recv);
     const uint32_t av = vec_ok(tab);
@@ -811,9 +828,10 @@ extern "C" int bine_batch_create(bine_vcomm_t comm, const bine_batch_item_t* ite
         if ((rc = check_range(it.recvbuf, partition_bytes(dp, BUF_RECV), "batch recvbuf"))) return rc;
         if (overlap(send, partition_bytes(dp, BUF_SEND), it.recvbuf, partition_bytes(dp, BUF_RECV)))
             return fail(BINE_ERR_ARG, "batch item buffers overlap");
+        const bool fuse = dp->cp.fused_ok && comm->cfg.exec != BINE_EXEC_STEP;
+        if (!fuse && (rc = ensure_tmp(comm, dp))) return rc;
         b->plans.push_back(dp);
         b->tabs.push_back(make_table(dp, send, it.recvbuf));
-        const bool fuse = dp->cp.fused_ok && comm->cfg.exec != BINE_EXEC_STEP;
         b->use_fused.push_back(fuse ? 1 : 0);
         if (!fuse) continue;
         auto lit = std::find_if(b->lists.begin(), b->lists.end(), [&](const bine_batch_s::TypeList& l) { return l.dtype == it.datatype; });

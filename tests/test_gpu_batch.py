@@ -1,0 +1,185 @@
+"""Batched execution on the GPU: uneven buckets and mixed collectives in one
+fused work list, eager and graph-replayed, against single calls and the oracle.
+
+Every batched output must equal, bit for bit, what the single collective
+call writes (itself checked against the oracle in test_gpu_parity.py), and a
+sampled column of each output must equal the oracle."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from bine_trees_fugaku_amd import Batch, VComm  # noqa: E402
+from bine_trees_fugaku_amd._lib import BineError  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def randn(shape, dtype, seed):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    return torch.randn(shape, dtype=dtype, device="cuda", generator=g)
+
+
+def randint(shape, dtype, seed):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    return torch.randint(-1000, 1001, shape, dtype=dtype, device="cuda", generator=g)
+
+
+def same_bits(a, b):
+    return torch.equal(a.view(torch.uint8), b.view(torch.uint8))
+
+
+def bucket_sizes(n, seed, lo=4096, hi=1 << 20):
+    rng = np.random.default_rng(seed)
+    return (np.exp(rng.uniform(np.log(lo), np.log(hi), n)).astype(np.int64) // 4).tolist()
+
+
+@pytest.mark.parametrize("mode", ["launch", "graph"])
+def test_uneven_buckets_equal_single_calls(gpu, mode):
+    P, dims = 64, [4, 4, 4]
+    sizes = bucket_sizes(24, 1) + [64, 65, 127, 4097]  # one element per block, ragged tails
+    sends = [randn((P, n), torch.float32, 100 + i) for i, n in enumerate(sizes)]
+    recvs = [torch.full_like(s, float("nan")) for s in sends]
+    with VComm(dims, exec="FUSED") as comm, VComm(dims, exec="STEP") as step:
+        b = Batch(comm, [{"coll": "allreduce", "sendbuf": s, "recvbuf": r, "count": n}
+                         for s, r, n in zip(sends, recvs, sizes)])
+        refs = [torch.empty_like(s) for s in sends]
+        for s, r in zip(sends, refs):
+            step.allreduce(s, r)
+        replays = 3 if mode == "graph" else 1
+        for k in range(replays):
+            for r in recvs:
+                r.fill_(float("nan"))  # eager work between replays
+            torch.cuda.synchronize()
+            b.launch_graph() if mode == "graph" else b.launch()
+            torch.cuda.synchronize()
+            for i, (r, ref) in enumerate(zip(recvs, refs)):
+                assert same_bits(r, ref), (k, i, sizes[i])
+        b.close()
+
+
+def test_uneven_buckets_against_oracle(gpu):
+    """Small buckets checked directly against the oracle (whole buckets, 0 ulp)."""
+    P, dims = 8, [2, 2, 2]
+    sizes = [8, 9, 15, 100, 1000, 4099]
+    sends = [randn((P, n), torch.float64, 7 + i) for i, n in enumerate(sizes)]
+    recvs = [torch.empty_like(s) for s in sends]
+    with VComm(dims, exec="FUSED") as comm:
+        b = Batch(comm, [{"coll": "allreduce", "sendbuf": s, "recvbuf": r, "count": n}
+                         for s, r, n in zip(sends, recvs, sizes)])
+        b.launch_graph()
+        torch.cuda.synchronize()
+        for s, r in zip(sends, recvs):
+            want = O.allreduce(s.cpu().numpy(), dims)
+            assert np.array_equal(r.cpu().numpy().view(np.int64), want.view(np.int64))
+        b.close()
+
+
+def test_every_tuning_gives_the_same_bits(gpu):
+    P, dims = 512, [8, 8, 8]
+    count = 4096 + 3
+    send = randn((P, count), torch.float64, 5)
+    recv = torch.empty_like(send)
+    with VComm(dims, exec="FUSED") as comm:
+        b = Batch(comm, [{"coll": "allreduce", "sendbuf": send, "recvbuf": recv, "count": count}])
+        b.set_tuning(1, 256, 0, 0)
+        b.launch()
+        ref = recv.clone()
+        for vpt, thr, pers, nt in [(2, 128, 0, 0), (4, 64, 0, 1), (4, 64, 4096, 1), (1, 512, 512, 0), (2, 64, 1024, 1)]:
+            recv.fill_(0)
+            b.set_tuning(vpt, thr, pers, nt)
+            b.launch_graph()
+            torch.cuda.synchronize()
+            assert same_bits(recv, ref), (vpt, thr, pers, nt)
+        b.autotune(iters=1)
+        recv.fill_(0)
+        b.launch_graph()
+        torch.cuda.synchronize()
+        assert same_bits(recv, ref)
+        b.close()
+
+
+def mixed_items(P, seed, scale=1):
+    """The 8x8x8 mixed workload: allreduce fp64 4 MiB per rank, alltoall int64
+    4 KiB per pair, bcast int32 64 KiB, scatter fp32 256 KiB blocks, gather int32
+    1 MiB blocks, reduce fp32 1 MiB (sizes divided by `scale`)."""
+    c_ar, c_a2a = 524288 // scale, 512 // scale
+    c_b, c_s, c_g, c_r = 16384 // scale, 65536 // scale, 262144 // scale, 262144 // scale
+    t = {
+        "allreduce": (randn((P, c_ar), torch.float64, seed), torch.empty((P, c_ar), dtype=torch.float64, device="cuda"), c_ar),
+        "alltoall": (randint((P, P * c_a2a), torch.int64, seed + 1), torch.empty((P, P * c_a2a), dtype=torch.int64, device="cuda"), c_a2a),
+        "bcast": (None, randint((P, c_b), torch.int32, seed + 2), c_b),
+        "scatter": (randn((P * c_s,), torch.float32, seed + 3), torch.empty((P, c_s), dtype=torch.float32, device="cuda"), c_s),
+        "gather": (randint((P, c_g), torch.int32, seed + 4), torch.empty((P * c_g,), dtype=torch.int32, device="cuda"), c_g),
+        "reduce": (randn((P, c_r), torch.float32, seed + 5), torch.empty((c_r,), dtype=torch.float32, device="cuda"), c_r),
+    }
+    return t
+
+
+def single_call(comm, coll, send, recv):
+    if coll == "allreduce":
+        comm.allreduce(send, recv)
+    elif coll == "alltoall":
+        comm.alltoall(send, recv)
+    elif coll == "bcast":
+        comm.bcast(recv)
+    elif coll == "scatter":
+        comm.scatter(send, recv)
+    elif coll == "gather":
+        comm.gather(send, recv)
+    elif coll == "reduce":
+        comm.reduce(send, recv)
+
+
+@pytest.mark.parametrize("scale", [64, 1])
+def test_mixed_collectives_8x8x8(gpu, scale):
+    P, dims = 512, [8, 8, 8]
+    t = mixed_items(P, 11, scale)
+    bcast_in = t["bcast"][1].clone()
+    with VComm(dims, exec="FUSED") as comm:
+        items = [{"coll": c, "sendbuf": s, "recvbuf": r, "count": n} for c, (s, r, n) in t.items()]
+        b = Batch(comm, items)
+        print(f"mixed batch scale 1/{scale}: {b.stats()}", flush=True)
+        b.launch_graph()
+        torch.cuda.synchronize()
+        got = {c: r.clone() for c, (s, r, n) in t.items()}
+        b.close()
+        # single calls on fresh outputs
+        t["bcast"][1].copy_(bcast_in)
+        for c, (s, r, n) in t.items():
+            if c != "bcast":
+                r.fill_(0)
+            single_call(comm, c, s, r)
+        torch.cuda.synchronize()
+        for c, (s, r, n) in t.items():
+            assert same_bits(got[c], r), c
+    # oracle on the small instance (whole buffers)
+    if scale == 64:
+        a = {c: (None if s is None else s.cpu().numpy(), r.cpu().numpy()) for c, (s, r, n) in t.items()}
+        assert np.array_equal(got["allreduce"].cpu().numpy(), O.allreduce(a["allreduce"][0], dims))
+        assert np.array_equal(got["alltoall"].cpu().numpy(), O.alltoall(a["alltoall"][0], P))
+        assert np.array_equal(got["bcast"].cpu().numpy(), O.bcast(bcast_in.cpu().numpy(), 0))
+        assert np.array_equal(got["scatter"].cpu().numpy(), O.scatter(a["scatter"][0], P, 0))
+        assert np.array_equal(got["gather"].cpu().numpy().reshape(-1), O.gather(a["gather"][0], 0).reshape(-1))
+        assert np.array_equal(got["reduce"].cpu().numpy().reshape(-1), O.reduce(a["reduce"][0], 0).reshape(-1))
+
+
+def test_batch_rejects_bad_items(gpu):
+    P = 8
+    s = randn((P, 16), torch.float32, 0)
+    r = torch.empty((P, 15), dtype=torch.float32, device="cuda")
+    with VComm([2, 2, 2]) as comm:
+        with pytest.raises(ValueError):
+            Batch(comm, [{"coll": "allreduce", "sendbuf": s, "recvbuf": r, "count": 16}])
+        with pytest.raises(ValueError):
+            Batch(comm, [{"coll": "allreduce", "sendbuf": s, "recvbuf": s.double(), "count": 16}])
+        small = randn((P, 7), torch.float32, 1)
+        with pytest.raises(BineError):  # count < P: lib/libbine.cc:1036 asserts the same
+            Batch(comm, [{"coll": "allreduce", "sendbuf": small, "recvbuf": torch.empty_like(small), "count": 7}])
+        r2 = torch.empty_like(s)
+        with pytest.raises(BineError):  # outputs of two items overlap
+            Batch(comm, [{"coll": "allreduce", "sendbuf": s, "recvbuf": r2, "count": 16},
+                         {"coll": "allreduce", "sendbuf": s, "recvbuf": r2, "count": 16}])

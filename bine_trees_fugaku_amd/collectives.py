@@ -14,6 +14,7 @@ Every call goes through the C ABI; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
+import os
 from typing import Optional, Sequence
 
 from . import _lib
@@ -83,16 +84,36 @@ class VComm:
     """A virtual communicator of P ranks living as HBM partitions of one GPU."""
 
     def __init__(self, dims: Sequence[int], num_ports: int = 1, exec: str = "AUTO", device: int = 0,
-                 from_env: bool = False, **overrides):
+                 from_env: bool = False, config: Optional[_lib.VCommConfig] = None, **overrides):
+        """config: a complete bine_vcomm_config_t to use instead of the other arguments."""
         self._lib = _lib.load()
-        self.config = make_config(dims, num_ports, exec, device, from_env, **overrides)
+        if config is not None:
+            self.config = _lib.VCommConfig()
+            ctypes.memmove(ctypes.byref(self.config), ctypes.byref(config), ctypes.sizeof(config))
+        else:
+            self.config = make_config(dims, num_ports, exec, device, from_env, **overrides)
         self.dims = [self.config.dimensions[i] for i in range(self.config.dimensions_num)]
+        self.num_ports = self.config.num_ports
         self.handle = ctypes.c_void_p()
         _lib.check(self._lib.bine_vcomm_create(ctypes.byref(self.config), ctypes.byref(self.handle)),
                    "bine_vcomm_create")
         size = ctypes.c_uint32()
         _lib.check(self._lib.bine_vcomm_size(self.handle, ctypes.byref(size)), "bine_vcomm_size")
         self.size = size.value
+
+    def with_exec(self, exec: str) -> "VComm":
+        """A new vcomm with this one's configuration and another execution mode."""
+        cfg = _lib.VCommConfig()
+        ctypes.memmove(ctypes.byref(cfg), ctypes.byref(self.config), ctypes.sizeof(cfg))
+        cfg.exec = EXEC[exec]
+        return VComm(self.dims, config=cfg)
+
+    @classmethod
+    def from_env(cls, exec: str = "AUTO", device: int = 0) -> "VComm":
+        """The torus and algorithms of the LIBBINE_* environment (LIBBINE_DIMENSIONS required)."""
+        if not os.environ.get("LIBBINE_DIMENSIONS"):
+            raise ValueError("LIBBINE_DIMENSIONS is not set")
+        return cls([1], exec=exec, device=device, from_env=True)
 
     def close(self):
         if self.handle:

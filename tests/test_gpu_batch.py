@@ -183,3 +183,19 @@ def test_batch_rejects_bad_items(gpu):
         with pytest.raises(BineError):  # outputs of two items overlap
             Batch(comm, [{"coll": "allreduce", "sendbuf": s, "recvbuf": r2, "count": 16},
                          {"coll": "allreduce", "sendbuf": s, "recvbuf": r2, "count": 16}])
+
+
+@pytest.mark.parametrize("collective,type_name,count", [
+    ("MPI_Allreduce", "INT32", 262144), ("MPI_Reduce_scatter", "FLOAT", 4096), ("MPI_Allgather", "INT", 1024),
+    ("MPI_Bcast", "CHAR", 65536), ("MPI_Alltoall", "INT32", 64), ("MPI_Scatter", "FLOAT", 1024),
+    ("MPI_Gather", "INT32", 1024), ("MPI_Reduce", "VOID", 65536), ("MPI_Allreduce", "DOUBLE", 4099)])
+def test_harness_runs_and_validates(gpu, collective, type_name, count, capsys):
+    """bench/bench.cc's loop on the GPU: record printed, timed output equal to per-step execution."""
+    from bine_trees_fugaku_amd import harness as H
+    with VComm([2, 2, 2]) as comm:
+        run = H.Run(collective, type_name, count, comm=comm)
+        samples = run.time(3, warmup=1)
+        assert len(samples) == 3 and all(s > 0 for s in samples)
+        assert run.validate()
+    rec = H.format_record(samples).splitlines()
+    assert rec[0] == "highest" and len(rec) == 5

@@ -111,3 +111,14 @@ def test_no_device_is_reported_not_crashed():
     _lib.check(L.bine_vcomm_config_init(ctypes.byref(cfg), 8), "init")
     h = ctypes.c_void_p()
     assert L.bine_vcomm_create(ctypes.byref(cfg), ctypes.byref(h)) == _lib.BINE_ERR_NO_DEVICE
+
+
+def test_remap_rank_export_against_golden_fixture():
+    import json
+    import os
+    from bine_trees_fugaku_amd import _lib
+    L = _lib.load()
+    with open(os.path.join(os.path.dirname(__file__), "golden", "remap_rank.json")) as f:
+        rows = json.load(f)["P_rank_remap"]
+    for P, r, want in rows:  # including non-powers of two, where the decimal 80000000 test acts
+        assert L.bine_remap_rank(r, P) == want, (P, r)

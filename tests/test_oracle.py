@@ -330,3 +330,15 @@ def test_sim_run_results_and_stats(dims):
     assert stats[1] == sum(2 ** (s / D) for s in range(L))
     assert stats[2] == sum(d / 2 ** (s + 1) for s, d in enumerate(dist))
     assert stats[4] == 1 - 2.0 ** -L
+
+
+def test_remap_rank_against_golden_fixture():
+    """Fixture written by tests/golden/make_remap_rank.py from the reference's
+    own simulation/remap_rank.c (runs where the reference tree is absent)."""
+    import json
+    path = os.path.join(os.path.dirname(__file__), "golden", "remap_rank.json")
+    with open(path) as f:
+        rows = json.load(f)["P_rank_remap"]
+    assert len(rows) > 150
+    for P, r, want in rows:
+        assert O.remap_rank(r, P) == want, (P, r)

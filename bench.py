@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-buckets", action="store_true", help="skip the uneven-bucket batched vs per-step run")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-collective batch run")
+    ap.add_argument("--no-linksim", action="store_true", help="skip the link-traffic simulation run")
+    ap.add_argument("--no-selector", action="store_true", help="skip training and scoring the selector")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--only", choices=["batched", "step", "copy"], default=None,
                     help="profiling aid: run only that part, --steps times, with a fixed launch shape")
@@ -217,6 +219,64 @@ def mixed_8x8x8(stream, seed=21):
     return out
 
 
+def linksim_tori(stream):
+    """Link-traffic simulation over the five tori 2x2x2, 4x4x4, 8x8x8, 64x64 and
+    32x256: every algorithm defined for each of the 8 collectives, one GPU pass of
+    integer atomics.  A sample is one (collective, algorithm, torus, message size)
+    point at the 26 sizes 2^3..2^28 bytes."""
+    import torch
+    from bine_trees_fugaku_amd import linksim as ls
+    tori = ([2, 2, 2], [4, 4, 4], [8, 8, 8], [64, 64], [32, 256])
+    cases = [(d, c, a) for d in tori for c, algos in ls.DEFINED.items() for a in algos
+             if not (a == "simpy" and len(set(d)) != 1)]
+    sim = ls.LinkSim(cases)
+    reps = 20
+    sim.run(stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        sim.run(stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    classes = crossings = 0
+    for i in range(len(cases)):
+        st = sim.stats(i)
+        classes += len(st.num)
+        crossings += int(st.sum_mult.sum())
+    sim.close()
+    samples = len(cases) * 26
+    return {"workload": "linksim_5_tori_all_algorithms", "cases": len(cases), "classes": classes,
+            "link_crossings_per_pass": crossings, "ms_per_pass": round(ms, 4),
+            "samples_per_s": round(samples / (ms * 1e-3), 1), "atomic_adds_per_s": round(crossings / (ms * 1e-3), 1)}
+
+
+def selector_regret():
+    """Learned selector: train the 3x128 network on the GPU on 199 training tori,
+    measure mean regret on the five held-out tori against exhaustive simulation."""
+    import numpy as np
+    from bine_trees_fugaku_amd import selector as sel
+    t0 = time.perf_counter()
+    train = sel.simulate(sel.training_tori(), sel.default_sizes(2))
+    test = sel.simulate(sel.HELD_OUT, sel.default_sizes(4))
+    t1 = time.perf_counter()
+    s = sel.Selector(seed=0)
+    s.fit(train, epochs=60)
+    t2 = time.perf_counter()
+    r = sel.regret(test, s.choose_rows(test)[0])
+    ch, unsure = s.choose_rows(test, margin=0.05)
+    rf = sel.regret(test, ch)
+    base = {name: round(float(sel.regret(test, sel.rule_choice_rows(test, rule)).mean()), 4)
+            for name, rule in (("reference_defaults", sel.reference_default_choice), ("fjmpi", sel.fjmpi_choice))}
+    s.close()
+    return {"train_rows": int(len(train.X)), "train_samples": train.samples, "simulate_s": round(t1 - t0, 2),
+            "train_s": round(t2 - t1, 2), "held_out_mean_regret": round(float(r.mean()), 4),
+            "held_out_p95_regret": round(float(np.percentile(r, 95)), 4),
+            "with_fallback_mean_regret": round(float(rf.mean()), 4), "fallback_rate": round(float(unsure.mean()), 4),
+            "baseline_mean_regret": base}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -294,6 +354,10 @@ def main():
     if not args.no_mixed:
         line["config"]["mixed_8x8x8"] = mixed_8x8x8(stream)
         torch.cuda.empty_cache()
+    if not args.no_linksim:
+        line["config"]["linksim"] = linksim_tori(stream)
+    if not args.no_selector:
+        line["config"]["selector"] = selector_regret()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -40,8 +40,11 @@ enum Epi : int {
 // C[M x N] = op(A)[M x K] * op(B)[K x N] with the epilogue above.
 // op(A)[m][k] = TA ? A[k * lda + m] : A[m * lda + k]
 // op(B)[k][n] = TB ? B[n * ldb + k] : B[k * ldb + n]
+// Split K: workgroup z covers k in [z * kc, min(K, (z + 1) * kc)) and, when
+// gridDim.z > 1, writes its partial product to C + z * M * ldc (EPI_PLAIN only;
+// sum_partials adds the slices in order).
 template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void gemm_f32(int M, int N, int K, const float* __restrict__ A, int lda,
+__global__ __launch_bounds__(256) void gemm_f32(int M, int N, int K, int kc, const float* __restrict__ A, int lda,
                                                 const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
                                                 const float* __restrict__ bias, const float* __restrict__ mask, int epi) {
     __shared__ float As[TK][TM + 4];  // As[k][m]
@@ -49,11 +52,14 @@ __global__ __launch_bounds__(256) void gemm_f32(int M, int N, int K, const float
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int kbeg = blockIdx.z * kc;
+    const int kend = kbeg + kc < K ? kbeg + kc : K;
+    C += (size_t)blockIdx.z * M * ldc;
     floatx4 acc[2][2];
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    for (int k0 = 0; k0 < K; k0 += TK) {
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
         // 64 x 16 elements of each operand, 4 per thread.
         for (int e = tid; e < TM * TK; e += 256) {
             int mm, kk;
@@ -66,7 +72,7 @@ __global__ __launch_bounds__(256) void gemm_f32(int M, int N, int K, const float
             }
             const int gm = m0 + mm, gk = k0 + kk;
             float v = 0.f;
-            if (gm < M && gk < K) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
+            if (gm < M && gk < kend) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
             As[kk][mm] = v;
         }
         for (int e = tid; e < TN * TK; e += 256) {
@@ -80,7 +86,7 @@ __global__ __launch_bounds__(256) void gemm_f32(int M, int N, int K, const float
             }
             const int gn = n0 + nn, gk = k0 + kk;
             float v = 0.f;
-            if (gn < N && gk < K) v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
+            if (gn < N && gk < kend) v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
             Bs[kk][nn] = v;
         }
         __syncthreads();
@@ -163,17 +169,30 @@ __global__ void reduce_partials(int nb, const float* __restrict__ partial, float
     if (loss_out) loss_out[0] = l / fmaxf(c, 1.f);
 }
 
-// db[j] = sum_i dZ[i][j]: one workgroup per 64 columns, rows split over 4 waves, fixed order.
-__global__ __launch_bounds__(256) void col_sum(int n, int cols, const float* __restrict__ dZ, float* __restrict__ db) {
+// Partial column sums: workgroup (x, y) sums rows [y * rc, min(n, (y + 1) * rc))
+// of 64 columns, rows split over 4 waves, into part[y * cols + col].
+__global__ __launch_bounds__(256) void col_sum(int n, int cols, int rc, const float* __restrict__ dZ,
+                                               float* __restrict__ part) {
     __shared__ float s[4][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = blockIdx.x * 64 + lane;
+    const int r0 = blockIdx.y * rc;
+    const int r1 = r0 + rc < n ? r0 + rc : n;
     float acc = 0.f;
     if (col < cols)
-        for (int i = wave; i < n; i += 4) acc += dZ[(size_t)i * cols + col];
+        for (int i = r0 + wave; i < r1; i += 4) acc += dZ[(size_t)i * cols + col];
     s[wave][lane] = acc;
     __syncthreads();
-    if (wave == 0 && col < cols) db[col] = (s[0][lane] + s[1][lane]) + (s[2][lane] + s[3][lane]);
+    if (wave == 0 && col < cols) part[(size_t)blockIdx.y * cols + col] = (s[0][lane] + s[1][lane]) + (s[2][lane] + s[3][lane]);
+}
+
+// out[i] = sum over z = 0 .. nz-1 of part[z * n + i], in that order.
+__global__ void sum_partials(int n, int nz, const float* __restrict__ part, float* __restrict__ out) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float acc = part[i];
+        for (int z = 1; z < nz; ++z) acc += part[(size_t)z * n + i];
+        out[i] = acc;
+    }
 }
 
 __global__ void adam_update(uint64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -191,8 +210,47 @@ __global__ void adam_update(uint64_t n, float* __restrict__ p, const float* __re
 template <bool TA, bool TB>
 void gemm(int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C, int ldc, const float* bias,
           const float* mask, int epi, hipStream_t st) {
-    dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
-    hipLaunchKernelGGL((gemm_f32<TA, TB>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, bias, mask, epi);
+    dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, 1);
+    hipLaunchKernelGGL((gemm_f32<TA, TB>), grid, dim3(256), 0, st, M, N, K, K, A, lda, B, ldb, C, ldc, bias, mask, epi);
+}
+
+// Number of K slices for a product whose output has `tiles` 64x64 tiles:
+// enough workgroups to cover the chip, slices of at least 128 rows, at most kMaxSplit.
+constexpr int kMaxSplit = 32;
+int k_slices(int K, int tiles) {
+    int nz = (512 + tiles - 1) / tiles;
+    nz = std::min(nz, std::max(1, K / 128));
+    return std::max(1, std::min(nz, kMaxSplit));
+}
+
+// C[M x N] = op(A) op(B) with K split into slices summed in a fixed order
+// (work: nz * M * N floats).
+template <bool TA, bool TB>
+void gemm_split(int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C, float* work,
+                hipStream_t st) {
+    const int tiles = ((N + TN - 1) / TN) * ((M + TM - 1) / TM);
+    const int nz = k_slices(K, tiles);
+    if (nz == 1) {
+        gemm<TA, TB>(M, N, K, A, lda, B, ldb, C, N, nullptr, nullptr, EPI_PLAIN, st);
+        return;
+    }
+    const int kc = ((K + nz - 1) / nz + TK - 1) / TK * TK;
+    const int nzr = (K + kc - 1) / kc;
+    dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, nzr);
+    hipLaunchKernelGGL((gemm_f32<TA, TB>), grid, dim3(256), 0, st, M, N, K, kc, A, lda, B, ldb, work, N, nullptr,
+                       nullptr, (int)EPI_PLAIN);
+    const int n = M * N;
+    hipLaunchKernelGGL(sum_partials, dim3(std::min(1024, (n + 255) / 256)), dim3(256), 0, st, n, nzr, work, C);
+}
+
+// db = column sums of dZ (n x cols), rows split into slices summed in a fixed order.
+void col_sums(int n, int cols, const float* dz, float* db, float* work, hipStream_t st) {
+    const int groups = (cols + 63) / 64;
+    const int nz = k_slices(n, groups);
+    const int rc = (n + nz - 1) / nz;
+    const int nzr = (n + rc - 1) / rc;
+    hipLaunchKernelGGL(col_sum, dim3(groups, nzr), dim3(256), 0, st, n, cols, rc, dz, work);
+    hipLaunchKernelGGL(sum_partials, dim3((cols + 255) / 256), dim3(256), 0, st, cols, nzr, work, db);
 }
 
 constexpr int kLossBlocks = 120;
@@ -212,9 +270,10 @@ struct bine_mlp_s {
     std::vector<float*> act;  // act[l]: n x dims[l+1] outputs of layer l (post-ReLU for hidden layers)
     float *dz0 = nullptr, *dz1 = nullptr;  // ping-pong n x max width
     float *partial = nullptr, *totals = nullptr;
+    float* work = nullptr;  // split-K partial products
     uint64_t step = 0;
     ~bine_mlp_s() {
-        for (float* p : {params, grads, adam_m, adam_v, dz0, dz1, partial, totals})
+        for (float* p : {params, grads, adam_m, adam_v, dz0, dz1, partial, totals, work})
             if (p) (void)hipFree(p);
         for (float* p : act)
             if (p) (void)hipFree(p);
@@ -266,8 +325,8 @@ void backward(bine_mlp_t m, const float* x, const float* t, const float* mask, i
         const int fi = m->dims[l], fout = m->dims[l + 1];
         const float* a_prev = l ? m->act[l - 1] : x;
         // dW_l = dZ_l^T A_{l-1}  (fout x fi), K = n
-        gemm<true, false>(fout, fi, n, dz, fout, a_prev, fi, m->grads + m->w_off[l], fi, nullptr, nullptr, EPI_PLAIN, st);
-        hipLaunchKernelGGL(col_sum, dim3((fout + 63) / 64), dim3(256), 0, st, n, fout, dz, m->grads + m->b_off[l]);
+        gemm_split<true, false>(fout, fi, n, dz, fout, a_prev, fi, m->grads + m->w_off[l], m->work, st);
+        col_sums(n, fout, dz, m->grads + m->b_off[l], m->work, st);
         if (l) {  // dZ_{l-1} = (dZ_l W_l) * (A_{l-1} > 0)
             gemm<false, false>(n, fi, fout, dz, fout, m->params + m->w_off[l], fi, other, fi, nullptr, m->act[l - 1],
                                EPI_RELU_MASK, st);
@@ -304,11 +363,11 @@ extern "C" int bine_mlp_create(const int32_t* dims, int32_t ndims, int32_t max_b
     }
     for (int i = 0; i < ndims; ++i) widest = std::max(widest, (int)dims[i]);
     m->nparams = off;
-    // He-uniform weights from splitmix64, zero biases.
+    // Weights uniform in +-1/sqrt(fan_in) from splitmix64, zero biases.
     std::vector<float> h(off, 0.f);
     uint64_t s = seed;
     for (int l = 0; l + 1 < ndims; ++l) {
-        const float bound = std::sqrt(6.f / (float)dims[l]);
+        const float bound = 1.f / std::sqrt((float)dims[l]);
         for (uint64_t i = 0; i < (uint64_t)dims[l] * dims[l + 1]; ++i) {
             const double u = (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0);
             h[m->w_off[l] + i] = (float)((2.0 * u - 1.0) * bound);
@@ -331,6 +390,9 @@ extern "C" int bine_mlp_create(const int32_t* dims, int32_t ndims, int32_t max_b
     }
     MHIP(hipMalloc(&m->dz0, sizeof(float) * (size_t)max_batch * widest));
     MHIP(hipMalloc(&m->dz1, sizeof(float) * (size_t)max_batch * widest));
+    size_t widest_w = 0;
+    for (int l = 0; l + 1 < ndims; ++l) widest_w = std::max(widest_w, (size_t)dims[l] * dims[l + 1]);
+    MHIP(hipMalloc(&m->work, sizeof(float) * kMaxSplit * std::max(widest_w, (size_t)widest)));
     MHIP(hipMalloc(&m->partial, sizeof(float) * 2 * kLossBlocks));
     MHIP(hipMalloc(&m->totals, sizeof(float) * 2));
     *out = m.release();

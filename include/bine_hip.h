@@ -289,7 +289,7 @@ int bine_sim_destroy(bine_sim_t sim);
  * Adam (beta 0.9 / 0.999, eps 1e-8).  Calls are stream-ordered. */
 typedef struct bine_mlp_s* bine_mlp_t;
 
-/* He-uniform weights from splitmix64(seed), zero biases; buffers for batches up to max_batch rows. */
+/* Weights uniform in +-1/sqrt(fan_in) from splitmix64(seed), zero biases; buffers for batches up to max_batch rows. */
 int bine_mlp_create(const int32_t* dims, int32_t ndims, int32_t max_batch, uint64_t seed, int device, bine_mlp_t* out);
 int bine_mlp_num_params(bine_mlp_t mlp, uint64_t* n);
 /* Flat parameters, layer by layer: W (outputs x inputs, row-major) then b. Synchronous. */

@@ -27,24 +27,27 @@ def cuda(a):
 
 @pytest.mark.parametrize("n", [1, 63, 300, 4096])
 def test_forward_and_gradients_match_numpy(gpu, n):
-    """Tolerance: |gpu - ref| <= 2e-4 * max|ref| (fp32 products accumulated
-    on the matrix cores vs float64)."""
+    """Tolerances, fp32 products accumulated on the matrix cores vs float64:
+    outputs within 2e-4 of their largest magnitude; every gradient entry within
+    1e-4 of its condition scale (the same sum with every term taken in absolute
+    value, so cancellation in a large sum is allowed for), while a wrong term
+    or index shifts entries by about the gradient's own size."""
     net = sel.MLP(DIMS, max_batch=4096, seed=n)
     p = net.params().astype(np.float64)
     x, t, m = batch(n, n)
     y = net.forward(cuda(x)).cpu().numpy()
     y_ref = mlp_ref.forward(p, DIMS, x.astype(np.float64))[-1]
     assert np.abs(y - y_ref).max() <= 2e-4 * np.abs(y_ref).max()
+    m[mlp_ref.near_kinks(p, DIMS, x)] = 0  # rows where fp32 may take the other side of a ReLU
     loss, g = net.gradients(cuda(x), cuda(t), cuda(m))
-    l_ref, g_ref = mlp_ref.loss_and_grads(p, DIMS, x, t, m)
+    l_ref, g_ref, g_scale = mlp_ref.loss_and_grads(p, DIMS, x, t, m, scale=True)
     assert float(loss.item()) == pytest.approx(l_ref, rel=1e-4)
-    g = g.cpu().numpy()
-    o = 0
-    for fi, fo in zip(DIMS[:-1], DIMS[1:]):  # per tensor, so small bias gradients are checked on their own scale
-        for size in (fi * fo, fo):
-            a, b = g[o:o + size], g_ref[o:o + size]
-            assert np.abs(a - b).max() <= 2e-4 * np.abs(b).max() + 1e-7, (fi, fo, size)
-            o += size
+    g = g.cpu().numpy().astype(np.float64)
+    err = np.abs(g - g_ref)
+    assert np.all(err <= 1e-4 * g_scale + 1e-12), float((err / (g_scale + 1e-30)).max())
+    # the check has teeth: the gradients are not small against their scale
+    live = g_scale > 0  # entries behind a dead ReLU are exactly 0 on both sides
+    assert live.mean() > 0.1 and np.median(np.abs(g_ref[live]) / g_scale[live]) > 1e-3
     net.close()
 
 
@@ -67,7 +70,7 @@ def test_adam_training_fits_a_smooth_function(gpu):
     loss = torch.zeros(1, device="cuda")
     net.train_step(X, T, M, 1e-3, loss)
     first = float(loss.item())
-    for _ in range(400):
+    for _ in range(1200):
         net.train_step(X, T, M, 1e-3, loss)
     assert float(loss.item()) < 0.1 * first
     net.close()

@@ -39,6 +39,17 @@ size_t or_dtsize(int dt) {
     }
 }
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+int or_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 int or_rho(int k) { return (k >= 0 && k < 20) ? RHOS[k] : 0; }
 
 /* lib/libbine_coll.cc:65-87 */
@@ -624,9 +635,11 @@ static int step_run(step_t* s) {
     size_t o = 0;
     for (size_t i = 0; i < s->ns; i++) {
         soff[i] = o;
-        memcpy(stage + o, s->sends[i].ptr, s->sends[i].bytes);
         o += s->sends[i].bytes;
     }
+    /* snapshot every send (messages are independent: one thread each) */
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long i = 0; i < (long)s->ns; i++) memcpy(stage + soff[i], s->sends[i].ptr, s->sends[i].bytes);
     int rc = (s->ns == s->nr) ? OR_OK : OR_ERR_MATCH;
     size_t* si = (size_t*)malloc(sizeof(size_t) * (s->ns + 1));
     size_t* ri = (size_t*)malloc(sizeof(size_t) * (s->nr + 1));
@@ -643,8 +656,13 @@ static int step_run(step_t* s) {
             rc = OR_ERR_MATCH;
             break;
         }
-        memcpy((void*)r->ptr, stage + soff[si[k]], snd->bytes);
         if (g_trace_on && snd->src != snd->dst) trace_msg(snd->src, snd->dst, snd->bytes);
+    }
+    /* deliver: every receive buffer is written by exactly one matched message */
+    if (rc == OR_OK) {
+#pragma omp parallel for schedule(dynamic, 16)
+        for (long k = 0; k < (long)s->nr; k++)
+            memcpy((void*)s->recvs[ri[k]].ptr, stage + soff[si[k]], s->sends[si[k]].bytes);
     }
     if (g_trace_on) g_trace_step++;
     free(si);
@@ -722,6 +740,8 @@ static int allreduce_b(const or_torus_t* t, int ports, int family, size_t count,
             }
             rc = step_run(&st);
             if (rc != OR_OK || coll == 1) continue;
+            /* every (port, rank) reduces its own partition: independent */
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
             for (int p = 0; p < ports; p++)
                 for (uint32_t r = 0; r < P; r++) {
                     size_t idx = (((size_t)p * P + r) * L + bstep) * P;

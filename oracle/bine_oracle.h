@@ -63,6 +63,9 @@ uint32_t or_remap_rank(uint32_t rank, uint32_t size);      /* lib/libbine_coll_a
 uint32_t or_get_nu(uint32_t rank, uint32_t size);          /* lib/libbine_coll_allgather.cc:1125-1158 */
 int or_remap_distance_doubling(uint32_t num);              /* lib/libbine_coll_alltoall.cc:401-409 */
 
+/* OpenMP threads the oracle's data movement and reductions use (1 without OpenMP). */
+int or_threads(void);
+
 /* ---- torus / schedule ------------------------------------------------ */
 int or_torus_init(or_torus_t* t, const uint32_t* dims, uint32_t ndims);
 void or_coord_from_id(const or_torus_t* t, int id, int* coord);   /* lib/libbine_common.cc:206-216 */

@@ -95,6 +95,7 @@ def lib():
             L.or_reverse.restype = u32
             L.or_sim_peer.argtypes = [i32, i32, pu32, u32, i32, ctypes.POINTER(ctypes.c_int)]
             L.or_sim_run.argtypes = [pu32, u32, i32, ctypes.POINTER(ctypes.c_int), vp, vp]
+            L.or_threads.argtypes = []
             L.or_trace_start.argtypes = []
             L.or_trace_stop.argtypes = []
             L.or_trace_stop.restype = ctypes.c_size_t
@@ -310,6 +311,11 @@ def sim_run(dims, data: np.ndarray, collective="ALLREDUCE", distances=JACOBSTHAL
 
 
 # ---- message trace and link routing -------------------------------------------
+
+def threads() -> int:
+    """OpenMP threads the oracle runs its data movement and reductions on."""
+    return int(lib().or_threads())
+
 
 def trace(fn, *args, **kwargs):
     """Run one oracle collective with the engine's message trace on.

@@ -11,6 +11,17 @@
  * Buffers: a "virtual-rank buffer" is P consecutive per-rank partitions of
  * `per_rank_bytes` each (rank r at base + r*per_rank_bytes), the same layout
  * the product's C-ABI uses.
+ *
+ * Pinning.  Every function is checked by the tests against closed forms,
+ * invariants, brute force, MPI semantics, the reference's own
+ * simulation/remap_rank.c (built under _ref/ and as a committed fixture) or
+ * the oracle's own message traces.  Parity unpinned: the fp association and
+ * block ownership that or_allreduce / or_get_tree / or_block_step produce on
+ * NON-power-of-two sizes (where compute_block_step's tie-break and the
+ * "disconnect myself" rule act, lib/libbine_common.cc:1499-1500, 1574, 2045)
+ * follow the code as read but are confirmed by no reference output; and
+ * MPI_Reduce_local for fp32/fp64 is taken as IEEE a + b (the system MPI is
+ * not part of the reference).
  */
 #ifndef BINE_ORACLE_H
 #define BINE_ORACLE_H

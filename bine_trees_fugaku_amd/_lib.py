@@ -23,7 +23,7 @@ EXPORTED = [
     "bine_compute_step_info", "bine_block_step", "bine_get_tree", "bine_remap_rank", "bine_reduce_local",
     "bine_plan_stats", "bine_plan_autotune", "bine_plan_dump", "bine_batch_create", "bine_batch_launch", "bine_batch_launch_graph",
     "bine_batch_autotune", "bine_batch_stats", "bine_batch_get_tuning", "bine_batch_set_tuning",
-    "bine_batch_destroy", "bine_sim_classes", "bine_sim_peers", "bine_sim_create", "bine_sim_run", "bine_sim_case_info",
+    "bine_batch_destroy", "bine_tuning_candidates", "bine_batch_time_shapes", "bine_sim_classes", "bine_sim_peers", "bine_sim_create", "bine_sim_run", "bine_sim_case_info",
     "bine_sim_class_stats", "bine_sim_link_counts", "bine_sim_destroy", "bine_mlp_create", "bine_mlp_num_params",
     "bine_mlp_get_params", "bine_mlp_set_params", "bine_mlp_forward", "bine_mlp_gradients", "bine_mlp_train_step",
     "bine_mlp_destroy", "bine_last_error",
@@ -111,6 +111,9 @@ def _sig(L):
     L.bine_batch_get_tuning.argtypes = [vp, pint, pint, pint, pint]
     L.bine_batch_set_tuning.argtypes = [vp, i32, i32, i32, i32]
     L.bine_batch_destroy.argtypes = [vp]
+    L.bine_tuning_candidates.argtypes = [i32, ctypes.POINTER(ctypes.c_int32), i32, pint]
+    L.bine_batch_time_shapes.argtypes = [vp, vp, i32, ctypes.POINTER(ctypes.c_int32), i32,
+                                         ctypes.POINTER(ctypes.c_float)]
     psc = ctypes.POINTER(SimCase)
     pu64 = ctypes.POINTER(ctypes.c_uint64)
     L.bine_sim_classes.argtypes = [psc, pu32, pu32, pu32, pu32, pu32, pu32, pu32]

@@ -349,6 +349,18 @@ class Batch:
                                                    ctypes.byref(n)), "bine_batch_get_tuning")
         return {"vpt": v.value, "threads": t.value, "persistent_wgs": p.value, "nontemporal": n.value}
 
+    def time_shapes(self, shapes, iters: int = 3, stream=None):
+        """Average ms of one launch at each launch shape (rows of vpt, threads,
+        persistent, nontemporal); the batch keeps its current shape."""
+        import numpy as np
+        arr = np.ascontiguousarray(np.asarray(shapes, np.int32).reshape(-1, 4))
+        ms = np.zeros(len(arr), np.float32)
+        _lib.check(self._lib.bine_batch_time_shapes(self.handle, VComm._stream(stream), iters,
+                                                    arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(arr),
+                                                    ms.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                   "bine_batch_time_shapes")
+        return ms
+
     def stats(self) -> dict:
         st = _lib.PlanStats()
         _lib.check(self._lib.bine_batch_stats(self.handle, ctypes.byref(st)), "bine_batch_stats")
@@ -364,3 +376,16 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def tuning_candidates(device: int = 0):
+    """The autotuner's launch shapes on a device: (n, 4) int32 rows of
+    vectors per thread, threads, persistent workgroups, non-temporal."""
+    import numpy as np
+    L = _lib.load()
+    n = ctypes.c_int()
+    _lib.check(L.bine_tuning_candidates(device, None, 0, ctypes.byref(n)), "bine_tuning_candidates")
+    out = np.zeros((n.value, 4), np.int32)
+    _lib.check(L.bine_tuning_candidates(device, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n.value,
+                                        ctypes.byref(n)), "bine_tuning_candidates")
+    return out

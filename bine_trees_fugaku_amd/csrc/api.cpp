@@ -944,40 +944,79 @@ extern "C" int bine_batch_set_tuning(bine_batch_t b, int32_t vpt, int32_t thread
     return relower ? batch_lower(b) : BINE_OK;
 }
 
+namespace {
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+int device_cus(int device) {
+    hipDeviceProp_t prop;
+    return hipGetDeviceProperties(&prop, device) == hipSuccess ? prop.multiProcessorCount : 256;
+}
+
+// ms[i]: average time of one batch launch at shapes[i] (one warm-up launch, then iters timed).
+int time_shapes(bine_batch_t b, hipStream_t st, int iters, const std::vector<LaunchShape>& shapes, float* ms) {
+    EventPair ev;
+    HIPCHK(hipEventCreate(&ev.a));
+    HIPCHK(hipEventCreate(&ev.b));
+    for (size_t i = 0; i < shapes.size(); ++i) {
+        const LaunchShape& c = shapes[i];
+        int rc = bine_batch_set_tuning(b, c.vpt, c.threads, c.persistent, c.nontemporal);
+        if (rc) return rc;
+        if ((rc = batch_run(b, st))) return rc;  // warm-up
+        HIPCHK(hipEventRecord(ev.a, st));
+        for (int k = 0; k < iters; ++k)
+            if ((rc = batch_run(b, st))) return rc;
+        HIPCHK(hipEventRecord(ev.b, st));
+        HIPCHK(hipEventSynchronize(ev.b));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, ev.a, ev.b));
+        ms[i] = t / (float)iters;
+    }
+    return BINE_OK;
+}
+
+}  // namespace
+
+extern "C" int bine_tuning_candidates(int device, int32_t* shapes, int cap, int* n) {
+    if (!n || cap < 0 || (cap > 0 && !shapes)) return fail(BINE_ERR_ARG, "bad arguments");
+    const std::vector<LaunchShape> c = tuning_candidates(device_cus(device));
+    *n = (int)c.size();
+    for (int i = 0; i < (int)c.size() && i < cap; ++i) {
+        shapes[4 * i] = c[i].vpt;
+        shapes[4 * i + 1] = c[i].threads;
+        shapes[4 * i + 2] = c[i].persistent;
+        shapes[4 * i + 3] = c[i].nontemporal;
+    }
+    return BINE_OK;
+}
+
+extern "C" int bine_batch_time_shapes(bine_batch_t b, void* stream, int iters, const int32_t* shapes, int n,
+                                      float* ms) {
+    if (!b || iters < 1 || n < 1 || !shapes || !ms) return fail(BINE_ERR_ARG, "bad arguments");
+    std::vector<LaunchShape> v;
+    for (int i = 0; i < n; ++i) v.push_back(LaunchShape{shapes[4 * i], shapes[4 * i + 1], shapes[4 * i + 2], shapes[4 * i + 3]});
+    HIPCHK(hipSetDevice(b->comm->cfg.device));
+    const LaunchShape keep = b->shape;
+    int rc = time_shapes(b, (hipStream_t)stream, iters, v, ms);
+    int rc2 = bine_batch_set_tuning(b, keep.vpt, keep.threads, keep.persistent, keep.nontemporal);
+    return rc ? rc : rc2;
+}
+
 extern "C" int bine_batch_autotune(bine_batch_t b, void* stream, int iters) {
     if (!b || iters < 1) return fail(BINE_ERR_ARG, "bad arguments");
     HIPCHK(hipSetDevice(b->comm->cfg.device));
-    hipStream_t st = (hipStream_t)stream;
-    int cus = 256;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, b->comm->cfg.device) == hipSuccess) cus = prop.multiProcessorCount;
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    float best = 1e30f;
-    LaunchShape bestc = b->shape;
-    for (const LaunchShape& c : tuning_candidates(cus)) {
-        int rc = bine_batch_set_tuning(b, c.vpt, c.threads, c.persistent, c.nontemporal);
-        if (rc) return rc;
-        rc = batch_run(b, st);  // warm-up
-        if (rc) return rc;
-        HIPCHK(hipEventRecord(e0, st));
-        for (int i = 0; i < iters; ++i) {
-            rc = batch_run(b, st);
-            if (rc) return rc;
-        }
-        HIPCHK(hipEventRecord(e1, st));
-        HIPCHK(hipEventSynchronize(e1));
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-            best = ms;
-            bestc = c;
-        }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return bine_batch_set_tuning(b, bestc.vpt, bestc.threads, bestc.persistent, bestc.nontemporal);
+    const std::vector<LaunchShape> cands = tuning_candidates(device_cus(b->comm->cfg.device));
+    std::vector<float> ms(cands.size(), 0.f);
+    int rc = time_shapes(b, (hipStream_t)stream, iters, cands, ms.data());
+    if (rc) return rc;
+    const size_t best = (size_t)(std::min_element(ms.begin(), ms.end()) - ms.begin());
+    return bine_batch_set_tuning(b, cands[best].vpt, cands[best].threads, cands[best].persistent, cands[best].nontemporal);
 }
 
 extern "C" int bine_batch_destroy(bine_batch_t b) {

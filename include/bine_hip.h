@@ -226,6 +226,12 @@ int bine_batch_get_tuning(bine_batch_t batch, int32_t* vpt, int32_t* threads, in
 int bine_batch_set_tuning(bine_batch_t batch, int32_t vpt, int32_t threads, int32_t persistent_wgs,
                           int32_t nontemporal);
 int bine_batch_destroy(bine_batch_t batch);
+/* The autotuner's candidate launch shapes on `device`: up to cap rows of
+ * {vectors per thread, threads, persistent workgroups, non-temporal}; *n = total. */
+int bine_tuning_candidates(int device, int32_t* shapes, int cap, int* n);
+/* Average time (ms) of one batch launch at each of n shapes (rows as above):
+ * one warm-up launch, then iters timed ones.  The batch keeps its shape. */
+int bine_batch_time_shapes(bine_batch_t batch, void* stream, int iters, const int32_t* shapes, int n, float* ms);
 
 /* ---- torus link-traffic simulation --------------------------------------
  * Replaces simulation/simulator.py (run_collectives :125-177, get_peer :70-76)

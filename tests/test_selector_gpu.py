@@ -135,3 +135,38 @@ def test_fallback_to_exhaustive_simulation(trained):
     algo, k, used = s.choose([8, 8, 8], "bcast", 1 << 20, margin=10.0)  # always unsure -> simulation
     T = sel.exhaustive([8, 8, 8], "bcast", 1 << 20)
     assert used and sel.OPTIONS.index((algo, k)) == int(np.argmin(T))
+
+
+# ---- learned launch shapes (trained on measured kernel timings) ----------------
+
+@pytest.fixture(scope="module")
+def shape_timings(gpu):
+    from bine_trees_fugaku_amd import shapes as S
+    w = S.default_workloads()
+    data = S.measure(w, iters=5, log=lambda i, k: print(f"shapes: {i}/{len(w)} {k}", flush=True) if i % 100 == 0 else None)
+    return S.split(data, frac=0.3, seed=0)
+
+
+def test_shape_model_regret_against_autotuner(shape_timings):
+    """Held-out workloads: the shape the network picks vs the fastest measured
+    shape (what the exhaustive autotuner finds), against the fixed default
+    shape and the best single shape fitted on the training workloads."""
+    from bine_trees_fugaku_amd import shapes as S
+    train, test = shape_timings
+    m = S.ShapeModel(len(train.shapes), seed=0)
+    m.fit(train, epochs=400)
+    choice = m.choose(test.X)
+    m.close()
+    r = S.regret(test, choice)
+    rows = np.arange(len(choice))
+    tw = test.T[rows, choice].sum() / test.T.min(axis=1).sum() - 1
+    default = [i for i, s in enumerate(test.shapes) if tuple(s) == (1, 256, 0, 0)][0]
+    bs = S.best_static(train)
+    r_def = S.regret(test, np.full(len(rows), default))
+    r_bs = S.regret(test, np.full(len(rows), bs))
+    tw_bs = test.T[:, bs].sum() / test.T.min(axis=1).sum() - 1
+    print(f"shape model: {len(train.X)} training / {len(test.X)} held-out workloads; mean regret {r.mean():.4f} "
+          f"(time-weighted {tw:.4f}); default shape {r_def.mean():.4f}; best static {r_bs.mean():.4f} "
+          f"(time-weighted {tw_bs:.4f})", flush=True)
+    assert r.mean() <= 0.08 and tw <= 0.08
+    assert r.mean() < 0.5 * r_bs.mean() and r.mean() < 0.5 * r_def.mean() and tw < 0.5 * tw_bs

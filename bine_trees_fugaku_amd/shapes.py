@@ -155,6 +155,14 @@ class ShapeModel:
     def choose(self, X: np.ndarray) -> np.ndarray:
         return np.argmin(self.predict(np.atleast_2d(X)), axis=1)
 
+    def tune(self, batch: Batch, coll: str, P: int, ds: int, count: int, shapes=None) -> tuple:
+        """Set a single-collective batch to the predicted fastest shape, without timing runs."""
+        shapes = tuning_candidates(self.device) if shapes is None else shapes
+        x = features(coll, P, ds, count, batch.stats())
+        vpt, threads, pers, nt = (int(v) for v in shapes[int(self.choose(x)[0])])
+        batch.set_tuning(vpt, threads, pers, nt)
+        return vpt, threads, pers, nt
+
     def close(self):
         self.net.close()
 

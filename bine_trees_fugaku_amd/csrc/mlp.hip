@@ -272,6 +272,7 @@ struct bine_mlp_s {
     float *partial = nullptr, *totals = nullptr;
     float* work = nullptr;  // split-K partial products
     uint64_t step = 0;
+    hipStream_t last = nullptr;  // stream of the last launch; host copies wait for it
     ~bine_mlp_s() {
         for (float* p : {params, grads, adam_m, adam_v, dz0, dz1, partial, totals, work})
             if (p) (void)hipFree(p);
@@ -408,6 +409,7 @@ extern "C" int bine_mlp_num_params(bine_mlp_t m, uint64_t* n) {
 extern "C" int bine_mlp_get_params(bine_mlp_t m, float* host) {
     if (!m || !host) return mfail(BINE_ERR_ARG, "NULL argument");
     MHIP(hipSetDevice(m->device));
+    MHIP(hipStreamSynchronize(m->last));
     MHIP(hipMemcpy(host, m->params, sizeof(float) * m->nparams, hipMemcpyDeviceToHost));
     return BINE_OK;
 }
@@ -415,6 +417,7 @@ extern "C" int bine_mlp_get_params(bine_mlp_t m, float* host) {
 extern "C" int bine_mlp_set_params(bine_mlp_t m, const float* host) {
     if (!m || !host) return mfail(BINE_ERR_ARG, "NULL argument");
     MHIP(hipSetDevice(m->device));
+    MHIP(hipStreamSynchronize(m->last));
     MHIP(hipMemcpy(m->params, host, sizeof(float) * m->nparams, hipMemcpyHostToDevice));
     const size_t pb = sizeof(float) * m->nparams;
     MHIP(hipMemset(m->adam_m, 0, pb));
@@ -429,6 +432,7 @@ extern "C" int bine_mlp_forward(bine_mlp_t m, const float* x, int32_t n, float* 
     if (!y) return mfail(BINE_ERR_ARG, "NULL output");
     MHIP(hipSetDevice(m->device));
     hipStream_t st = (hipStream_t)stream;
+    m->last = st;
     forward(m, x, n, st);
     MHIP(hipMemcpyAsync(y, m->act.back(), sizeof(float) * (size_t)n * m->dims.back(), hipMemcpyDeviceToDevice, st));
     MHIP(hipGetLastError());
@@ -442,6 +446,7 @@ extern "C" int bine_mlp_gradients(bine_mlp_t m, const float* x, const float* t, 
     if (!t || !mask || !grads) return mfail(BINE_ERR_ARG, "NULL target, mask or gradient buffer");
     MHIP(hipSetDevice(m->device));
     hipStream_t st = (hipStream_t)stream;
+    m->last = st;
     forward(m, x, n, st);
     backward(m, x, t, mask, n, loss, st);
     MHIP(hipMemcpyAsync(grads, m->grads, sizeof(float) * m->nparams, hipMemcpyDeviceToDevice, st));
@@ -457,6 +462,7 @@ extern "C" int bine_mlp_train_step(bine_mlp_t m, const float* x, const float* t,
     if (!(lr > 0.f)) return mfail(BINE_ERR_ARG, "learning rate must be positive");
     MHIP(hipSetDevice(m->device));
     hipStream_t st = (hipStream_t)stream;
+    m->last = st;
     forward(m, x, n, st);
     backward(m, x, t, mask, n, loss, st);
     m->step += 1;

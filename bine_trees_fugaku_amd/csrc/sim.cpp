@@ -126,6 +126,7 @@ struct bine_sim_s {
     uint64_t* d_sum = nullptr;
     uint64_t mult_len = 0;
     uint32_t ncls = 0, max_p = 0;
+    hipStream_t last = nullptr;  // stream of the last run; queries wait for it
     ~bine_sim_s() {
         for (void* p : {(void*)d_cases, (void*)d_classes, (void*)d_mult, (void*)d_hops, (void*)d_max, (void*)d_sum})
             if (p) (void)hipFree(p);
@@ -214,6 +215,7 @@ extern "C" int bine_sim_run(bine_sim_t s, void* stream) {
     SHIP(hipSetDevice(s->device));
     SHIP(launch_sim(s->d_cases, s->d_classes, s->ncls, s->max_p, s->d_mult, s->mult_len, s->d_hops, s->d_max, s->d_sum,
                     (hipStream_t)stream));
+    s->last = (hipStream_t)stream;
     return BINE_OK;
 }
 
@@ -228,6 +230,7 @@ extern "C" int bine_sim_class_stats(bine_sim_t s, int i, uint32_t* num, uint32_t
                                     uint32_t* max_mult, uint64_t* sum_mult, uint32_t* max_hops) {
     if (!s || i < 0 || i >= (int)s->cases.size()) return sfail(BINE_ERR_ARG, "bad case index");
     SHIP(hipSetDevice(s->device));
+    SHIP(hipStreamSynchronize(s->last));
     const uint32_t first = s->class_first[i];
     const size_t nc = s->cases[i].classes.size();
     for (size_t c = 0; c < nc; ++c) {
@@ -246,6 +249,7 @@ extern "C" int bine_sim_link_counts(bine_sim_t s, int i, uint32_t cls, uint32_t*
     if (!s || i < 0 || i >= (int)s->cases.size() || !counts) return sfail(BINE_ERR_ARG, "bad arguments");
     if (cls >= s->cases[i].classes.size()) return sfail(BINE_ERR_ARG, "bad class index");
     SHIP(hipSetDevice(s->device));
+    SHIP(hipStreamSynchronize(s->last));
     const uint64_t off = s->mult_off[s->class_first[i] + cls];
     SHIP(hipMemcpy(counts, s->d_mult + off, s->nlinks[i] * sizeof(uint32_t), hipMemcpyDeviceToHost));
     return BINE_OK;

@@ -210,7 +210,9 @@ typedef struct {
 
 typedef struct bine_batch_s* bine_batch_t;
 
-/* Compile `n` collectives on one vcomm into one fused work list. */
+/* Compile `n` collectives on one vcomm into one fused work list.  The buffers
+ * are bound here and must stay allocated, and the vcomm alive, until
+ * bine_batch_destroy. */
 int bine_batch_create(bine_vcomm_t comm, const bine_batch_item_t* items, int n, bine_batch_t* out);
 /* Run the whole batch: one persistent launch per element type present. */
 int bine_batch_launch(bine_batch_t batch, void* stream);
@@ -277,7 +279,7 @@ int bine_sim_create(const bine_sim_case_t* cases, int n, int device, bine_sim_t*
 /* Count every class of every case: one GPU pass of integer atomics. */
 int bine_sim_run(bine_sim_t sim, void* stream);
 int bine_sim_case_info(bine_sim_t sim, int i, uint32_t* nclasses, uint32_t* nlinks);
-/* Per class of case i (synchronous copies): share, repetitions, the most
+/* Per class of case i (synchronous copies after the last run's stream): share, repetitions, the most
  * loaded link's message count, the sum of link crossings, the longest route. */
 int bine_sim_class_stats(bine_sim_t sim, int i, uint32_t* num, uint32_t* den, uint32_t* rep, uint32_t* max_mult,
                          uint64_t* sum_mult, uint32_t* max_hops);

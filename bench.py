@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-collective batch run")
     ap.add_argument("--no-linksim", action="store_true", help="skip the link-traffic simulation run")
     ap.add_argument("--no-selector", action="store_true", help="skip training and scoring the selector")
+    ap.add_argument("--no-small", action="store_true", help="skip the 2x2x2 and 4x4x4 allreduce configurations")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--only", choices=["batched", "step", "copy"], default=None,
                     help="profiling aid: run only that part, --steps times, with a fixed launch shape")
@@ -221,6 +222,59 @@ def mixed_8x8x8(stream, seed=21):
     return out
 
 
+def small_configs(stream, cpu=True):
+    """Allreduce on 2x2x2 int32 (262144 elements per rank, uniform integers in
+    [-1000, 1000]) and on 4x4x4 fp32 (4M elements per rank, N(0,1)): GPU per-step,
+    GPU fused single call, GPU batched graph, and the oracle on the host (one run
+    of the whole configuration).  GB/s = 2*P*S / time."""
+    import numpy as np
+    import torch
+    from bine_trees_fugaku_amd import Batch, VComm
+    out = {}
+    for name, dims, dt, count in (("allreduce_2x2x2_int32_1MiB", [2, 2, 2], torch.int32, 262144),
+                                  ("allreduce_4x4x4_fp32_16MiB", [4, 4, 4], torch.float32, 4194304)):
+        Pc = int(np.prod(dims))
+        g = torch.Generator(device="cuda")
+        g.manual_seed(Pc)
+        if dt == torch.int32:
+            x = torch.randint(-1000, 1001, (Pc, count), dtype=dt, device="cuda", generator=g)
+        else:
+            x = torch.randn((Pc, count), dtype=dt, device="cuda", generator=g)
+        y = torch.empty_like(x)
+        b_alg = 2 * Pc * count * x.element_size()
+        res = {"algorithmic_bytes": b_alg}
+        with torch.cuda.stream(stream):
+            for exec in ("STEP", "FUSED"):
+                c = VComm(dims, exec=exec)
+                c.autotune("allreduce", x, y, count, exec=exec, iters=2, stream=stream)
+                w, _ = time_launches(lambda: c.allreduce(x, y, stream=stream), stream, 20, 3)
+                res[f"{exec.lower()}_ms"] = round(w, 4)
+                res[f"{exec.lower()}_GBps"] = round(b_alg / (w * 1e-3) / 1e9, 1)
+                c.close()
+            c = VComm(dims, exec="FUSED")
+            bt = Batch(c, [{"coll": "allreduce", "sendbuf": x, "recvbuf": y, "count": count}])
+            bt.autotune(iters=2, stream=stream)
+            w, _ = time_launches(lambda: bt.launch_graph(stream), stream, 20, 3)
+            res["batched_ms"] = round(w, 4)
+            res["batched_GBps"] = round(b_alg / (w * 1e-3) / 1e9, 1)
+            bt.close()
+            c.close()
+        torch.cuda.synchronize()
+        if cpu:
+            from oracle import oracle as O
+            xh = x.cpu().numpy()
+            t0 = time.perf_counter()
+            ref = O.allreduce(xh, dims)
+            el = time.perf_counter() - t0
+            res["cpu_oracle_s"] = round(el, 4)
+            res["cpu_oracle_GBps"] = round(b_alg / el / 1e9, 3)
+            res["cpu_threads"] = O.threads()
+            res["bitwise_equal_oracle"] = bool(np.array_equal(y.cpu().numpy().view(np.uint8), ref.view(np.uint8)))
+        out[name] = res
+        del x, y
+    return out
+
+
 def linksim_tori(stream):
     """Link-traffic simulation over the five tori 2x2x2, 4x4x4, 8x8x8, 64x64 and
     32x256: every algorithm defined for each of the 8 collectives, one GPU pass of
@@ -355,6 +409,9 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_mixed:
         line["config"]["mixed_8x8x8"] = mixed_8x8x8(stream)
+        torch.cuda.empty_cache()
+    if not args.no_small:
+        line["config"]["small_configs"] = small_configs(stream, cpu=not args.no_cpu_baseline)
         torch.cuda.empty_cache()
     if not args.no_linksim:
         line["config"]["linksim"] = linksim_tori(stream)

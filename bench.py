@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--no-linksim", action="store_true", help="skip the link-traffic simulation run")
     ap.add_argument("--no-selector", action="store_true", help="skip training and scoring the selector")
     ap.add_argument("--no-small", action="store_true", help="skip the 2x2x2 and 4x4x4 allreduce configurations")
+    ap.add_argument("--no-collectives", action="store_true", help="skip the per-collective 8x8x8 runs")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--only", choices=["batched", "step", "copy"], default=None,
                     help="profiling aid: run only that part, --steps times, with a fixed launch shape")
@@ -275,6 +276,61 @@ def small_configs(stream, cpu=True):
     return out
 
 
+def per_collective_8x8x8(stream):
+    """Each collective alone on 8x8x8 (P=512), fused single call, autotuned:
+    GB/s of its compulsory bytes (read every input once, write every output once).
+    allreduce/reduce_scatter/allgather move 4 MiB per rank, alltoall 4 KiB per
+    rank pair (int64), bcast/scatter/gather/reduce 1 MiB per rank."""
+    import torch
+    from bine_trees_fugaku_amd import VComm
+    Pc = 512
+    g = torch.Generator(device="cuda")
+    g.manual_seed(99)
+    out = {}
+    M = 1 << 20
+    # (name, dtype, count, send elems, recv elems, compulsory bytes, call)
+    cases = [
+        ("allreduce", torch.float32, M, Pc * M, Pc * M, 2 * Pc * M * 4),
+        ("reduce_scatter", torch.float32, M // Pc, Pc * M, M, (Pc + 1) * M * 4),
+        ("allgather", torch.float32, M // Pc, M, Pc * M, (M + Pc * M) * 4),
+        ("alltoall", torch.int64, 512, Pc * Pc * 512, Pc * Pc * 512, 2 * Pc * Pc * 512 * 8),
+        ("bcast", torch.int32, M // 4, 0, Pc * (M // 4), (Pc + 1) * M),
+        ("scatter", torch.float32, M // 4, Pc * (M // 4), Pc * (M // 4), 2 * Pc * M),
+        ("gather", torch.int32, M // 4, Pc * (M // 4), Pc * (M // 4), 2 * Pc * M),
+        ("reduce", torch.float32, M // 4, Pc * (M // 4), M // 4, (Pc + 1) * M),
+    ]
+    with torch.cuda.stream(stream):
+        comm = VComm(DIMS, exec="FUSED")
+        for name, dt, count, ns, nr, b_alg in cases:
+            send = (torch.randn(ns, device="cuda", generator=g).to(dt) if ns else None)
+            recv = torch.randn(nr, device="cuda", generator=g).to(dt)
+            if name == "allreduce":
+                fn = lambda: comm.allreduce(send, recv, stream=stream)  # noqa: E731
+            elif name == "reduce_scatter":
+                fn = lambda: comm.reduce_scatter(send, recv, stream=stream)  # noqa: E731
+            elif name == "allgather":
+                fn = lambda: comm.allgather(send, recv, stream=stream)  # noqa: E731
+            elif name == "alltoall":
+                fn = lambda: comm.alltoall(send, recv, stream=stream)  # noqa: E731
+            elif name == "bcast":
+                fn = lambda: comm.bcast(recv, stream=stream)  # noqa: E731
+            elif name == "scatter":
+                fn = lambda: comm.scatter(send, recv, stream=stream)  # noqa: E731
+            elif name == "gather":
+                fn = lambda: comm.gather(send, recv, stream=stream)  # noqa: E731
+            else:
+                fn = lambda: comm.reduce(send, recv, stream=stream)  # noqa: E731
+            comm.autotune(name, send, recv, count, exec="FUSED", iters=2, stream=stream)
+            w, ev = time_launches(fn, stream, 10, 2)
+            out[name] = {"algorithmic_bytes": int(b_alg), "ms": round(ev, 4),
+                         "GBps": round(b_alg / (ev * 1e-3) / 1e9, 1),
+                         "pct_hbm_peak": round(100 * b_alg / (ev * 1e-3) / 1e9 / HBM_PEAK_GBPS, 1)}
+            del send, recv
+        comm.close()
+    torch.cuda.synchronize()
+    return out
+
+
 def linksim_tori(stream):
     """Link-traffic simulation over the five tori 2x2x2, 4x4x4, 8x8x8, 64x64 and
     32x256: every algorithm defined for each of the 8 collectives, one GPU pass of
@@ -412,6 +468,9 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_small:
         line["config"]["small_configs"] = small_configs(stream, cpu=not args.no_cpu_baseline)
+        torch.cuda.empty_cache()
+    if not args.no_collectives:
+        line["config"]["collectives_8x8x8"] = per_collective_8x8x8(stream)
         torch.cuda.empty_cache()
     if not args.no_linksim:
         line["config"]["linksim"] = linksim_tori(stream)

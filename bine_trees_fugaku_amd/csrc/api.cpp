@@ -25,6 +25,8 @@ namespace bine {
 hipError_t launch_gred(int dtype, int max_lg, const GredItem* items, uint32_t n, const PoolEntry* pool,
                        const BufTable& tab, const BufTable* tabs, bool batch, uint32_t allow_vec,
                        const LaunchShape& sh, hipStream_t st);
+hipError_t launch_grun(uint32_t esize, const GredItem* items, uint32_t n, const PoolEntry* pool, const BufTable& tab,
+                       const BufTable* tabs, bool batch, uint32_t allow_vec, const LaunchShape& sh, hipStream_t st);
 hipError_t launch_reduce_local(int dtype, const void* in, void* inout, uint64_t n, hipStream_t st);
 }  // namespace bine
 
@@ -195,6 +197,15 @@ int get_plan(bine_vcomm_t comm, const CollRequest& req, int dtype, DevPlan** out
     return BINE_OK;
 }
 
+// Kernel launches issued for a lowered work list (a launch range with both
+// kinds of items issues the reduction kernel and the copy-run kernel).
+uint64_t kernel_launches(const Lowered& low) {
+    uint64_t n = 0;
+    for (size_t i = 0; i < low.launches.size(); ++i)
+        n += (low.launches[i].second > low.runs[i] ? 1 : 0) + (low.runs[i] ? 1 : 0);
+    return n;
+}
+
 // Allocate the per-step schedule's temporaries (the fused schedule needs none).
 // Every rank keeps the reference's full-size temporary, so per-step execution
 // of large rooted collectives can need P times the reference's per-process memory.
@@ -258,11 +269,19 @@ uint32_t vec_ok(const BufTable& t) {
 }
 
 int launch_arrays(const DevArrays& d, int dtype, const BufTable& tab, uint32_t allow_vec, hipStream_t st) {
-    for (const auto& l : d.low.launches) {
-        if (!l.second) continue;
-        hipError_t e = launch_gred(dtype, d.max_lg, d.items + l.first, l.second, d.pool, tab, nullptr, false,
-                                   allow_vec, d.shape, st);
-        if (e != hipSuccess) return fail(BINE_ERR_HIP, std::string("gather-reduce launch: ") + hipGetErrorString(e));
+    for (size_t li = 0; li < d.low.launches.size(); ++li) {
+        const auto& l = d.low.launches[li];
+        const uint32_t nrun = d.low.runs[li], nred = l.second - nrun;
+        if (nred) {
+            hipError_t e = launch_gred(dtype, d.max_lg, d.items + l.first, nred, d.pool, tab, nullptr, false,
+                                       allow_vec, d.shape, st);
+            if (e != hipSuccess) return fail(BINE_ERR_HIP, std::string("gather-reduce launch: ") + hipGetErrorString(e));
+        }
+        if (nrun) {
+            hipError_t e = launch_grun((uint32_t)dtsize(dtype), d.items + l.first + nred, nrun, d.pool, tab, nullptr,
+                                       false, allow_vec, d.shape, st);
+            if (e != hipSuccess) return fail(BINE_ERR_HIP, std::string("copy-run launch: ") + hipGetErrorString(e));
+        }
     }
     return BINE_OK;
 }
@@ -514,8 +533,7 @@ extern "C" int bine_plan_stats(bine_vcomm_t comm, int32_t coll, int count, bine_
     if (exec == BINE_EXEC_AUTO) exec = dp->cp.fused_ok ? BINE_EXEC_FUSED : BINE_EXEC_STEP;
     if (exec == BINE_EXEC_FUSED && !dp->cp.fused_ok) return fail(BINE_ERR_UNSUPPORTED, "schedule cannot be fused: " + dp->cp.fused_why);
     const DevArrays& d = exec == BINE_EXEC_FUSED ? dp->fused : dp->step;
-    out->launches = 0;
-    for (const auto& l : d.low.launches) out->launches += l.second ? 1 : 0;
+    out->launches = kernel_launches(d.low);
     out->items = d.low.items.size();
     out->bytes_read = d.low.bytes_read;
     out->bytes_written = d.low.bytes_written;
@@ -736,6 +754,7 @@ struct bine_batch_s {
         std::vector<std::vector<Group>> groups;  // one launch
         std::vector<uint32_t> group_bucket;      // bucket of each group
         DevArrays arr;
+        LaunchShape shape = default_shape();     // each element type is tuned on its own
     };
     std::vector<TypeList> lists;
     BufTable* d_tabs = nullptr;
@@ -754,31 +773,39 @@ struct bine_batch_s {
 
 namespace {
 
-int batch_lower(bine_batch_t b) {
-    for (auto& l : b->lists) {
+int lower_list(bine_batch_s::TypeList& l) {
+    {
         l.arr.release();
         const size_t ds = dtsize(l.dtype);
-        // lower each bucket separately so every item carries its bucket index
+        // lower each batch item's groups (a consecutive span with one bucket) together,
+        // so every work item carries its bucket index and runs of copies can form
         Lowered all;
+        std::vector<GredItem> run_items;  // appended after the others: they have a kernel of their own
         int max_lg = 0;
-        for (size_t gi = 0; gi < l.groups[0].size(); ++gi) {
+        const auto& gs = l.groups[0];
+        for (size_t g0 = 0; g0 < gs.size();) {
+            size_t g1 = g0 + 1;
+            while (g1 < gs.size() && l.group_bucket[g1] == l.group_bucket[g0]) ++g1;
             Lowered one;
-            std::vector<std::vector<Group>> v(1, std::vector<Group>{l.groups[0][gi]});
-            lower_groups(v, shallow_tile_elems(b->shape, ds), (int)(16 / ds), l.group_bucket[gi], ds, &one,
-                         deep_tile_elems(b->shape, ds), split_lg(b->shape));
+            std::vector<std::vector<Group>> v(1, std::vector<Group>(gs.begin() + g0, gs.begin() + g1));
+            lower_groups(v, shallow_tile_elems(l.shape, ds), (int)(16 / ds), l.group_bucket[g0], ds, &one,
+                         deep_tile_elems(l.shape, ds), split_lg(l.shape));
             const uint32_t base = (uint32_t)all.pool.size();
             for (auto it : one.items) {
                 it.leaf_off += base;
                 it.out_off += base;
-                all.items.push_back(it);
+                ((it.flags & ITEM_RUN) ? run_items : all.items).push_back(it);
             }
             all.pool.insert(all.pool.end(), one.pool.begin(), one.pool.end());
             all.bytes_read += one.bytes_read;
             all.bytes_written += one.bytes_written;
             all.max_leaves = std::max(all.max_leaves, one.max_leaves);
-            max_lg = std::max<int>(max_lg, l.groups[0][gi].lg);
+            for (size_t gi = g0; gi < g1; ++gi) max_lg = std::max<int>(max_lg, gs[gi].lg);
+            g0 = g1;
         }
+        all.items.insert(all.items.end(), run_items.begin(), run_items.end());
         all.launches.push_back({0, (uint32_t)all.items.size()});
+        all.runs.push_back((uint32_t)run_items.size());
         l.arr.low = std::move(all);
         l.arr.max_lg = max_lg;
         if (!l.arr.low.items.empty()) {
@@ -791,12 +818,35 @@ int batch_lower(bine_batch_t b) {
     return BINE_OK;
 }
 
+int batch_lower(bine_batch_t b) {
+    for (auto& l : b->lists) {
+        int rc = lower_list(l);
+        if (rc) return rc;
+    }
+    return BINE_OK;
+}
+
+// Launch one element type's work list: the reduction kernel, then the copy runs.
+int run_list(bine_batch_t b, const bine_batch_s::TypeList& l, hipStream_t st) {
+    if (l.arr.low.items.empty()) return BINE_OK;
+    const uint32_t nrun = l.arr.low.runs[0], nred = (uint32_t)l.arr.low.items.size() - nrun;
+    if (nred) {
+        hipError_t e = launch_gred(l.dtype, l.arr.max_lg, l.arr.items, nred, l.arr.pool, BufTable{}, b->d_tabs, true,
+                                   1, l.shape, st);
+        if (e != hipSuccess) return fail(BINE_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
+    }
+    if (nrun) {
+        hipError_t e = launch_grun((uint32_t)dtsize(l.dtype), l.arr.items + nred, nrun, l.arr.pool, BufTable{},
+                                   b->d_tabs, true, 1, l.shape, st);
+        if (e != hipSuccess) return fail(BINE_ERR_HIP, std::string("batch copy-run launch: ") + hipGetErrorString(e));
+    }
+    return BINE_OK;
+}
+
 int batch_run(bine_batch_t b, hipStream_t st) {
     for (auto& l : b->lists) {
-        if (l.arr.low.items.empty()) continue;
-        hipError_t e = launch_gred(l.dtype, l.arr.max_lg, l.arr.items, (uint32_t)l.arr.low.items.size(), l.arr.pool,
-                                   BufTable{}, b->d_tabs, true, 1, b->shape, st);
-        if (e != hipSuccess) return fail(BINE_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
+        int rc = run_list(b, l, st);
+        if (rc) return rc;
     }
     // collectives whose schedule cannot be fused run their step launches after
     for (size_t i = 0; i < b->items.size(); ++i) {
@@ -895,7 +945,7 @@ extern "C" int bine_batch_stats(bine_batch_t b, bine_plan_stats_t* out) {
     if (!b || !out) return fail(BINE_ERR_ARG, "NULL argument");
     std::memset(out, 0, sizeof(*out));
     for (auto& l : b->lists) {
-        out->launches += l.arr.low.items.empty() ? 0 : 1;
+        out->launches += kernel_launches(l.arr.low);
         out->items += l.arr.low.items.size();
         out->bytes_read += l.arr.low.bytes_read;
         out->bytes_written += l.arr.low.bytes_written;
@@ -904,7 +954,7 @@ extern "C" int bine_batch_stats(bine_batch_t b, bine_plan_stats_t* out) {
     for (size_t i = 0; i < b->items.size(); ++i) {
         if (b->use_fused[i]) continue;
         const DevArrays& d = b->plans[i]->step;
-        for (const auto& l : d.low.launches) out->launches += l.second ? 1 : 0;
+        out->launches += kernel_launches(d.low);
         out->items += d.low.items.size();
         out->bytes_read += d.low.bytes_read;
         out->bytes_written += d.low.bytes_written;
@@ -938,10 +988,17 @@ extern "C" int bine_batch_set_tuning(bine_batch_t b, int32_t vpt, int32_t thread
         persistent_wgs < 0)
         return fail(BINE_ERR_ARG, "bad launch shape");
     LaunchShape sh{vpt, threads, persistent_wgs, nontemporal ? 1 : 0};
-    const bool relower = sh.vpt != b->shape.vpt || sh.threads != b->shape.threads;
     b->shape = sh;
     drop_graph(b);
-    return relower ? batch_lower(b) : BINE_OK;
+    for (auto& l : b->lists) {
+        const bool relower = sh.vpt != l.shape.vpt || sh.threads != l.shape.threads;
+        l.shape = sh;
+        if (relower) {
+            int rc = lower_list(l);
+            if (rc) return rc;
+        }
+    }
+    return BINE_OK;
 }
 
 namespace {
@@ -1011,12 +1068,50 @@ extern "C" int bine_batch_time_shapes(bine_batch_t b, void* stream, int iters, c
 extern "C" int bine_batch_autotune(bine_batch_t b, void* stream, int iters) {
     if (!b || iters < 1) return fail(BINE_ERR_ARG, "bad arguments");
     HIPCHK(hipSetDevice(b->comm->cfg.device));
+    hipStream_t st = (hipStream_t)stream;
     const std::vector<LaunchShape> cands = tuning_candidates(device_cus(b->comm->cfg.device));
-    std::vector<float> ms(cands.size(), 0.f);
-    int rc = time_shapes(b, (hipStream_t)stream, iters, cands, ms.data());
-    if (rc) return rc;
-    const size_t best = (size_t)(std::min_element(ms.begin(), ms.end()) - ms.begin());
-    return bine_batch_set_tuning(b, cands[best].vpt, cands[best].threads, cands[best].persistent, cands[best].nontemporal);
+    // every element type's list is timed alone at every candidate and keeps its fastest shape
+    const size_t nl = b->lists.size();
+    std::vector<float> best(nl, 1e30f);
+    std::vector<LaunchShape> pick(nl, default_shape());
+    EventPair ev;
+    HIPCHK(hipEventCreate(&ev.a));
+    HIPCHK(hipEventCreate(&ev.b));
+    drop_graph(b);
+    for (const LaunchShape& c : cands) {
+        for (size_t i = 0; i < nl; ++i) {
+            auto& l = b->lists[i];
+            const bool relower = c.vpt != l.shape.vpt || c.threads != l.shape.threads;
+            l.shape = c;
+            int rc = relower ? lower_list(l) : BINE_OK;
+            if (rc) return rc;
+            if ((rc = run_list(b, l, st))) return rc;  // warm-up
+            HIPCHK(hipEventRecord(ev.a, st));
+            for (int k = 0; k < iters; ++k)
+                if ((rc = run_list(b, l, st))) return rc;
+            HIPCHK(hipEventRecord(ev.b, st));
+            HIPCHK(hipEventSynchronize(ev.b));
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, ev.a, ev.b));
+            if (t < best[i]) {
+                best[i] = t;
+                pick[i] = c;
+            }
+        }
+    }
+    size_t dominant = 0;
+    for (size_t i = 0; i < nl; ++i) {
+        auto& l = b->lists[i];
+        const bool relower = pick[i].vpt != l.shape.vpt || pick[i].threads != l.shape.threads;
+        l.shape = pick[i];
+        if (relower) {
+            int rc = lower_list(l);
+            if (rc) return rc;
+        }
+        if (best[i] > best[dominant]) dominant = i;
+    }
+    if (nl) b->shape = pick[dominant];
+    return BINE_OK;
 }
 
 extern "C" int bine_batch_destroy(bine_batch_t b) {

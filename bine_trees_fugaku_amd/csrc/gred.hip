@@ -20,10 +20,13 @@
 
 namespace bine {
 
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+
+// 16 bytes of elements of type T, held as one 128-bit register quad.
 template <typename T>
 struct Vec {
     static constexpr int N = 16 / sizeof(T);
-    T v[N];
+    u4 raw;
 };
 
 template <typename T>
@@ -45,30 +48,39 @@ __device__ __forceinline__ int8_t add1<int8_t>(int8_t a, int8_t b) {
 
 template <typename T>
 __device__ __forceinline__ Vec<T> vadd(const Vec<T>& a, const Vec<T>& b) {
-    Vec<T> r;
+    T x[Vec<T>::N], y[Vec<T>::N];
+    __builtin_memcpy(x, &a.raw, 16);
+    __builtin_memcpy(y, &b.raw, 16);
 #pragma unroll
-    for (int i = 0; i < Vec<T>::N; ++i) r.v[i] = add1<T>(a.v[i], b.v[i]);
+    for (int i = 0; i < Vec<T>::N; ++i) x[i] = add1<T>(x[i], y[i]);
+    Vec<T> r;
+    __builtin_memcpy(&r.raw, x, 16);
+    return r;
+}
+
+// 16 bytes added lane by lane (mod 256) on four 32-bit words: the low seven
+// bits of each byte cannot carry out of the byte, the top bit is the xor of
+// the carry into it with both operands' top bits.
+template <>
+__device__ __forceinline__ Vec<int8_t> vadd<int8_t>(const Vec<int8_t>& a, const Vec<int8_t>& b) {
+    const u4 x = a.raw, y = b.raw;
+    Vec<int8_t> r;
+    r.raw = ((x & 0x7F7F7F7Fu) + (y & 0x7F7F7F7Fu)) ^ ((x ^ y) & 0x80808080u);
     return r;
 }
 
 template <typename T, bool NT>
 __device__ __forceinline__ Vec<T> vload(const char* p) {
-    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-    u4 x;
-    if (NT) x = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p));
-    else x = *reinterpret_cast<const u4*>(p);
     Vec<T> r;
-    __builtin_memcpy(&r, &x, 16);
+    if (NT) r.raw = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p));
+    else r.raw = *reinterpret_cast<const u4*>(p);
     return r;
 }
 
 template <typename T, bool NT>
 __device__ __forceinline__ void vstore(char* p, const Vec<T>& v) {
-    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-    u4 x;
-    __builtin_memcpy(&x, &v, 16);
-    if (NT) __builtin_nontemporal_store(x, reinterpret_cast<u4*>(p));
-    else *reinterpret_cast<u4*>(p) = x;
+    if (NT) __builtin_nontemporal_store(v.raw, reinterpret_cast<u4*>(p));
+    else *reinterpret_cast<u4*>(p) = v.raw;
 }
 
 // Buffer address of a (buffer, rank, element offset) entry.  The buffer is
@@ -87,20 +99,29 @@ struct Table {
 };
 
 // Evaluate the balanced tree over 2^lg leaves; `load(k)` fetches leaf k.
-template <typename W, int MAXS, typename AddF, typename LoadF>
+// Leaves are taken in groups of G (4 or 8), each reduced pairwise, and the
+// group sums are folded with a binary counter: the association is the same
+// balanced tree for either G; G = 4 halves the loads in flight per thread.
+template <typename W, int MAXS, int G, typename AddF, typename LoadF>
 __device__ __forceinline__ W tree_eval(uint32_t lg, AddF add, LoadF load) {
     if (lg == 0) return load(0);
     if (lg == 1) return add(load(0), load(1));
     if (lg == 2) return add(add(load(0), load(1)), add(load(2), load(3)));
     W stk[MAXS];
-    const uint32_t S = lg - 3;
+    const uint32_t S = lg - (G == 8 ? 3 : 2);
     const uint32_t ngroups = 1u << S;
     W res;
     for (uint32_t j = 0; j < ngroups; ++j) {
-        const uint32_t k = j << 3;
-        W x0 = load(k + 0), x1 = load(k + 1), x2 = load(k + 2), x3 = load(k + 3);
-        W x4 = load(k + 4), x5 = load(k + 5), x6 = load(k + 6), x7 = load(k + 7);
-        W v = add(add(add(x0, x1), add(x2, x3)), add(add(x4, x5), add(x6, x7)));
+        const uint32_t k = j * G;
+        W v;
+        if (G == 8) {
+            W x0 = load(k + 0), x1 = load(k + 1), x2 = load(k + 2), x3 = load(k + 3);
+            W x4 = load(k + 4), x5 = load(k + 5), x6 = load(k + 6), x7 = load(k + 7);
+            v = add(add(add(x0, x1), add(x2, x3)), add(add(x4, x5), add(x6, x7)));
+        } else {
+            W x0 = load(k + 0), x1 = load(k + 1), x2 = load(k + 2), x3 = load(k + 3);
+            v = add(add(x0, x1), add(x2, x3));
+        }
         // binary-counter carry: level l holds the sum of the previous 2^l groups
         bool carry = true;
 #pragma unroll
@@ -117,6 +138,55 @@ __device__ __forceinline__ W tree_eval(uint32_t lg, AddF add, LoadF load) {
         res = v;
     }
     return res;
+}
+
+// ITEM_RUN items: k copies; copy j reads first leaf + j * step and writes first
+// output + j * step.  Element-type agnostic: raw 16-byte vectors (esize only
+// scales offsets).  A kernel of its own so the reduction kernel's register
+// allocation is untouched.
+template <int VPT, bool BATCH, bool NT>
+__global__ void __launch_bounds__(512) grun_kernel(const GredItem* __restrict__ items, uint32_t nitems,
+                                                   const PoolEntry* __restrict__ pool, BufTable tab,
+                                                   const BufTable* __restrict__ tabs, uint32_t allow_vec,
+                                                   uint32_t esize) {
+    for (uint32_t ii = blockIdx.x; ii < nitems; ii += gridDim.x) {
+        const GredItem it = items[ii];
+        const BufTable& bt = BATCH ? tabs[it.bucket] : tab;
+        const Table t{bt.base[0], bt.base[1], bt.base[2], bt.base[3], bt.stride[0], bt.stride[1], bt.stride[2], bt.stride[3]};
+        const PoolEntry l0 = pool[it.leaf_off], ls = pool[it.leaf_off + 1];
+        const PoolEntry o0 = pool[it.out_off], os = pool[it.out_off + 1];
+        const uint64_t delta = it.delta;
+        const bool vec = (it.flags & ITEM_VEC) && allow_vec;
+        for (uint32_t j = 0; j < it.pad; ++j) {
+            const char* src = t.at(l0.rb + ls.rb * j, (uint64_t)l0.off + (uint64_t)ls.off * j + delta, esize);
+            char* dst = t.at(o0.rb + os.rb * j, (uint64_t)o0.off + (uint64_t)os.off * j + delta, esize);
+            if (vec) {
+                const uint32_t nvec = (uint32_t)(((uint64_t)it.n * esize) / 16);
+                for (uint32_t base = 0; base < nvec; base += blockDim.x * VPT) {
+                    u4 a[VPT];
+#pragma unroll
+                    for (int u = 0; u < VPT; ++u) {
+                        const uint32_t vi = base + threadIdx.x + u * blockDim.x;
+                        if (vi < nvec) {
+                            const u4* q = reinterpret_cast<const u4*>(src + (uint64_t)vi * 16);
+                            a[u] = NT ? __builtin_nontemporal_load(q) : *q;
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < VPT; ++u) {
+                        const uint32_t vi = base + threadIdx.x + u * blockDim.x;
+                        if (vi < nvec) {
+                            u4* q = reinterpret_cast<u4*>(dst + (uint64_t)vi * 16);
+                            if (NT) __builtin_nontemporal_store(a[u], q);
+                            else *q = a[u];
+                        }
+                    }
+                }
+            } else {
+                for (uint64_t b = threadIdx.x; b < (uint64_t)it.n * esize; b += blockDim.x) dst[b] = src[b];
+            }
+        }
+    }
 }
 
 template <typename T, int VPT>
@@ -181,7 +251,7 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
                     }
                     return r;
                 };
-                Pack<T, VPT> part = tree_eval<Pack<T, VPT>, MAXS>(lgs, addp, loadp);
+                Pack<T, VPT> part = tree_eval<Pack<T, VPT>, MAXS, (VPT == 4 ? 4 : 8)>(lgs, addp, loadp);
 #pragma unroll
                 for (int u = 0; u < VPT; ++u) lds[((size_t)wave * VPT + u) * 64 + lane] = part.x[u];
                 __syncthreads();
@@ -229,7 +299,7 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
                     }
                     return r;
                 };
-                Pack<T, VPT> s = tree_eval<Pack<T, VPT>, MAXS>(lg, addp, loadp);
+                Pack<T, VPT> s = tree_eval<Pack<T, VPT>, MAXS, (VPT == 4 ? 4 : 8)>(lg, addp, loadp);
                 for (uint32_t o = 0; o < it.nouts; ++o) {
                     const PoolEntry e = outs[o];
                     char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
@@ -245,7 +315,7 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
                     const PoolEntry e = leaves[k];
                     return *reinterpret_cast<const T*>(t.at(e.rb, (uint64_t)e.off + delta + e0, sizeof(T)));
                 };
-                T s = tree_eval<T, MAXS>(lg, adds, loads);
+                T s = tree_eval<T, MAXS, 8>(lg, adds, loads);
                 for (uint32_t o = 0; o < it.nouts; ++o) {
                     const PoolEntry e = outs[o];
                     *reinterpret_cast<T*>(t.at(e.rb, (uint64_t)e.off + delta + e0, sizeof(T))) = s;
@@ -291,8 +361,8 @@ static hipError_t launch_v(int max_lg, const GredItem* items, uint32_t n, const 
     case 2:
         return small ? launch_t<T, 2, 6, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st)
                      : launch_t<T, 2, 11, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st);
-    default:  // four vectors per thread only with the shallow stack (the deep one spills)
-        return small ? launch_t<T, 4, 6, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st)
+    default:  // four vectors per thread: groups of 4 leaves, one more stack level; deep trees run with two
+        return small ? launch_t<T, 4, 7, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st)
                      : launch_t<T, 2, 11, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st);
     }
 }
@@ -314,6 +384,31 @@ hipError_t launch_gred(int dtype, int max_lg, const GredItem* items, uint32_t n,
     default: return hipErrorInvalidValue;
     }
 #undef BINE_DISPATCH
+}
+
+hipError_t launch_grun(uint32_t esize, const GredItem* items, uint32_t n, const PoolEntry* pool, const BufTable& tab,
+                       const BufTable* tabs, bool batch, uint32_t allow_vec, const LaunchShape& sh, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    if (sh.threads < 64 || sh.threads > 512 || (sh.threads & (sh.threads - 1))) return hipErrorInvalidValue;
+    uint32_t grid = sh.persistent > 0 ? (uint32_t)sh.persistent : n;
+    if (grid > n) grid = n;
+#define BINE_RUN(V, B, N)                                                                                     \
+    hipLaunchKernelGGL((grun_kernel<V, B, N>), dim3(grid), dim3(sh.threads), 0, st, items, n, pool, tab, tabs, \
+                       allow_vec, esize)
+#define BINE_RUN_V(B, N)           \
+    switch (sh.vpt) {              \
+    case 1: BINE_RUN(1, B, N); break; \
+    case 2: BINE_RUN(2, B, N); break; \
+    default: BINE_RUN(4, B, N); break; \
+    }
+    if (batch) {
+        if (sh.nontemporal) { BINE_RUN_V(true, true) } else { BINE_RUN_V(true, false) }
+    } else {
+        if (sh.nontemporal) { BINE_RUN_V(false, true) } else { BINE_RUN_V(false, false) }
+    }
+#undef BINE_RUN_V
+#undef BINE_RUN
+    return hipGetLastError();
 }
 
 hipError_t launch_reduce_local(int dtype, const void* in, void* inout, uint64_t n, hipStream_t st) {

@@ -14,7 +14,13 @@ struct PoolEntry {
 
 inline PoolEntry make_entry(uint32_t buf, uint32_t rank, uint32_t off) { return PoolEntry{rank | (buf << 28), off}; }
 
-enum : uint16_t { ITEM_VEC = 1 };
+enum : uint16_t {
+    ITEM_VEC = 1,  // every leaf/output address is 16-byte aligned
+    ITEM_RUN = 2,  // a run of `pad` one-leaf, one-output copies (see GredItem)
+};
+
+// Longest run of copies merged into one item.
+constexpr uint32_t kMaxRun = 32;
 
 // One tile of one group: elements [delta, delta + n) of the group.
 struct alignas(16) GredItem {
@@ -26,7 +32,9 @@ struct alignas(16) GredItem {
     uint32_t delta;
     uint32_t n;
     uint32_t bucket;    // buffer table index (batches); 0 for single calls
-    uint32_t pad;
+    uint32_t pad;       // ITEM_RUN: number of copies k >= 2.  The pool holds {first leaf, leaf step}
+                        // at leaf_off and {first output, output step} at out_off; copy j reads
+                        // first leaf + j * step (rank and offset both advance) and writes likewise.
 };
 static_assert(sizeof(GredItem) == 32, "GredItem layout");
 

@@ -1651,8 +1651,60 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
                   size_t dtsize, Lowered* out, int64_t deep_tile_elems, int deep_lg) {
     for (const auto& gs : launches) {
         const uint32_t first = (uint32_t)out->items.size();
+        // Open run of aligned single copies: consecutive groups whose leaf and
+        // output advance by one fixed (rank, offset) step become one ITEM_RUN item.
+        int64_t run_item = -1;
+        uint32_t run_len = 0;
+        PoolEntry run_leaf{}, run_out{}, step_leaf{}, step_out{};
+        auto single_copy = [&](const Group& g) {
+            return g.lg == 0 && g.leaves.size() == 1 && g.outs.size() == 1 && g.n <= shallow_tile && g.n % vec == 0 &&
+                   g.leaves[0].off % vec == 0 && g.outs[0].off % vec == 0;
+        };
+        // step b - a with non-negative rank and offset parts in the same buffer
+        auto step_of = [](const PoolEntry& a, const PoolEntry& b, PoolEntry* st) {
+            if ((a.rb >> 28) != (b.rb >> 28) || (b.rb & 0x0FFFFFFFu) < (a.rb & 0x0FFFFFFFu) || b.off < a.off) return false;
+            *st = PoolEntry{(b.rb & 0x0FFFFFFFu) - (a.rb & 0x0FFFFFFFu), b.off - a.off};
+            return true;
+        };
+        auto advance = [](const PoolEntry& a, const PoolEntry& st, uint32_t k) {
+            return PoolEntry{a.rb + st.rb * k, a.off + st.off * k};
+        };
         for (const Group& g : gs) {
             if (g.n <= 0) continue;
+            if (single_copy(g) && run_item >= 0 && run_len < kMaxRun && out->items[run_item].n == (uint32_t)g.n) {
+                bool ok;
+                if (run_len == 1) {
+                    PoolEntry sl, so;
+                    ok = step_of(run_leaf, g.leaves[0], &sl) && step_of(run_out, g.outs[0], &so) &&
+                         (sl.rb | sl.off | so.rb | so.off) != 0;
+                    if (ok) {
+                        step_leaf = sl;
+                        step_out = so;
+                    }
+                } else {
+                    const PoolEntry l = advance(run_leaf, step_leaf, run_len), o = advance(run_out, step_out, run_len);
+                    ok = l.rb == g.leaves[0].rb && l.off == g.leaves[0].off && o.rb == g.outs[0].rb && o.off == g.outs[0].off;
+                }
+                if (ok) {
+                    ++run_len;
+                    GredItem& it = out->items[run_item];
+                    if (run_len == 2) {  // the head's two pool entries are the last ones: rewrite them as a run
+                        out->pool.resize(it.leaf_off);
+                        out->pool.push_back(run_leaf);
+                        out->pool.push_back(step_leaf);
+                        out->pool.push_back(run_out);
+                        out->pool.push_back(step_out);
+                        it.out_off = it.leaf_off + 2;
+                        it.flags |= ITEM_RUN;
+                    }
+                    it.pad = run_len;
+                    out->bytes_read += (uint64_t)g.n * dtsize;
+                    out->bytes_written += (uint64_t)g.n * dtsize;
+                    continue;
+                }
+            }
+            run_item = -1;
+            run_len = 0;
             const int64_t tile_elems = (g.lg >= deep_lg && deep_tile_elems > 0) ? deep_tile_elems : shallow_tile;
             const uint32_t leaf_off = (uint32_t)out->pool.size();
             out->pool.insert(out->pool.end(), g.leaves.begin(), g.leaves.end());
@@ -1686,12 +1738,22 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
                 d += body;
             }
             for (; d < g.n; d += tile_elems) emit(d, std::min(tile_elems, g.n - d), false);
+            if (single_copy(g)) {  // exactly one aligned item: a run may start here
+                run_item = (int64_t)out->items.size() - 1;
+                run_len = 1;
+                run_leaf = g.leaves[0];
+                run_out = g.outs[0];
+            }
             const uint64_t nl = g.leaves.size();
             out->bytes_read += (uint64_t)g.n * nl * dtsize;
             out->bytes_written += (uint64_t)g.n * g.outs.size() * dtsize;
             out->max_leaves = std::max<uint64_t>(out->max_leaves, nl);
         }
+        // run items go last: they are executed by their own kernel
+        auto mid = std::stable_partition(out->items.begin() + first, out->items.end(),
+                                         [](const GredItem& it) { return !(it.flags & ITEM_RUN); });
         out->launches.push_back({first, (uint32_t)out->items.size() - first});
+        out->runs.push_back((uint32_t)(out->items.end() - mid));
     }
 }
 

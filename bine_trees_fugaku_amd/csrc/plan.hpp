@@ -106,6 +106,7 @@ struct Lowered {
     std::vector<GredItem> items;
     std::vector<PoolEntry> pool;
     std::vector<std::pair<uint32_t, uint32_t>> launches;  // (first item, item count)
+    std::vector<uint32_t> runs;  // per launch: how many of its items are ITEM_RUN items (they come last)
     uint64_t bytes_read = 0, bytes_written = 0, max_leaves = 0;
 };
 // tile_elems: elements per work item; vec_elems: elements per 16-byte vector.

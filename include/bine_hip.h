@@ -218,11 +218,14 @@ int bine_batch_create(bine_vcomm_t comm, const bine_batch_item_t* items, int n, 
 int bine_batch_launch(bine_batch_t batch, void* stream);
 /* Same work captured once into a HIP graph and replayed. */
 int bine_batch_launch_graph(bine_batch_t batch, void* stream);
-/* Time candidate (vector width, workgroup, persistence) shapes on the batch and keep the fastest. */
+/* Time candidate (vector width, workgroup, persistence, streaming) shapes and keep the fastest, chosen
+ * separately for each element type's work list. */
 int bine_batch_autotune(bine_batch_t batch, void* stream, int iters);
 int bine_batch_stats(bine_batch_t batch, bine_plan_stats_t* out);
-/* Chosen launch shape: 16-byte vectors per thread (1, 2, 4), threads per workgroup (64..512, power of
- * two), persistent grid (0 = one workgroup per item), streaming (non-temporal) loads and stores. */
+/* Launch shape: 16-byte vectors per thread (1, 2, 4), threads per workgroup (64..512, power of two),
+ * persistent grid (0 = one workgroup per item), streaming (non-temporal) loads and stores.  get returns
+ * the last shape set, or after autotuning the shape of the list that takes longest; set applies one
+ * shape to every list. */
 int bine_batch_get_tuning(bine_batch_t batch, int32_t* vpt, int32_t* threads, int32_t* persistent_wgs,
                           int32_t* nontemporal);
 int bine_batch_set_tuning(bine_batch_t batch, int32_t vpt, int32_t threads, int32_t persistent_wgs,

@@ -3,7 +3,9 @@
 It executes the work items that libbine_hip.so would upload (obtained with
 ``plan_dump``) on host arrays: each item reduces its 2^lg leaves with the
 balanced pairwise tree ((l0+l1)+(l2+l3))+... -- the association the HIP
-kernel implements -- and stores the result to every output.  Items of one
+kernel implements -- and stores the result to every output.  ITEM_RUN items
+are runs of single copies whose source and destination advance by a fixed
+(rank, offset) step.  Items of one
 launch touch disjoint data (the plan compiler proves it), so running them in
 order is equivalent to the GPU's concurrent execution.
 """
@@ -12,6 +14,7 @@ from __future__ import annotations
 import numpy as np
 
 BUF_SEND, BUF_RECV, BUF_TMP, BUF_TMP2 = 0, 1, 2, 3
+ITEM_RUN = 2
 
 
 def alloc_buffers(meta, dtype, send=None, recv=None):
@@ -45,6 +48,13 @@ def run(meta, items, pool, launches, bufs):
             for it in items[first:first + cnt]:
                 n = int(it["n"])
                 d = int(it["delta"])
+                if int(it["flags"]) & ITEM_RUN:  # k copies advancing by a fixed (rank, offset) step
+                    lf, ls = pool[it["leaf_off"]], pool[it["leaf_off"] + 1]
+                    of, os_ = pool[it["out_off"]], pool[it["out_off"] + 1]
+                    for j in range(int(it["pad"])):
+                        src = view(int(lf["rb"]) + j * int(ls["rb"]), int(lf["off"]) + j * int(ls["off"]) + d, n).copy()
+                        view(int(of["rb"]) + j * int(os_["rb"]), int(of["off"]) + j * int(os_["off"]) + d, n)[...] = src
+                    continue
                 nl = 1 << int(it["lg"])
                 lv = pool[it["leaf_off"]:it["leaf_off"] + nl]
                 vals = [view(e["rb"], int(e["off"]) + d, n).copy() for e in lv]

@@ -157,33 +157,50 @@ __global__ void __launch_bounds__(512) grun_kernel(const GredItem* __restrict__ 
         const PoolEntry o0 = pool[it.out_off], os = pool[it.out_off + 1];
         const uint64_t delta = it.delta;
         const bool vec = (it.flags & ITEM_VEC) && allow_vec;
-        for (uint32_t j = 0; j < it.pad; ++j) {
-            const char* src = t.at(l0.rb + ls.rb * j, (uint64_t)l0.off + (uint64_t)ls.off * j + delta, esize);
-            char* dst = t.at(o0.rb + os.rb * j, (uint64_t)o0.off + (uint64_t)os.off * j + delta, esize);
+        // two copies per pass, so every thread keeps 2 * VPT 16-byte loads in flight
+        for (uint32_t j = 0; j < it.pad; j += 2) {
+            const uint32_t j1 = j + 1 < it.pad ? j + 1 : j;  // an odd tail copies its last block twice
+            const char* s0 = t.at(l0.rb + ls.rb * j, (uint64_t)l0.off + (uint64_t)ls.off * j + delta, esize);
+            const char* s1 = t.at(l0.rb + ls.rb * j1, (uint64_t)l0.off + (uint64_t)ls.off * j1 + delta, esize);
+            char* d0 = t.at(o0.rb + os.rb * j, (uint64_t)o0.off + (uint64_t)os.off * j + delta, esize);
+            char* d1 = t.at(o0.rb + os.rb * j1, (uint64_t)o0.off + (uint64_t)os.off * j1 + delta, esize);
             if (vec) {
                 const uint32_t nvec = (uint32_t)(((uint64_t)it.n * esize) / 16);
                 for (uint32_t base = 0; base < nvec; base += blockDim.x * VPT) {
-                    u4 a[VPT];
+                    u4 a[VPT], c[VPT];
 #pragma unroll
                     for (int u = 0; u < VPT; ++u) {
                         const uint32_t vi = base + threadIdx.x + u * blockDim.x;
                         if (vi < nvec) {
-                            const u4* q = reinterpret_cast<const u4*>(src + (uint64_t)vi * 16);
-                            a[u] = NT ? __builtin_nontemporal_load(q) : *q;
+                            const u4* q0 = reinterpret_cast<const u4*>(s0 + (uint64_t)vi * 16);
+                            const u4* q1 = reinterpret_cast<const u4*>(s1 + (uint64_t)vi * 16);
+                            a[u] = NT ? __builtin_nontemporal_load(q0) : *q0;
+                            c[u] = NT ? __builtin_nontemporal_load(q1) : *q1;
                         }
                     }
 #pragma unroll
                     for (int u = 0; u < VPT; ++u) {
                         const uint32_t vi = base + threadIdx.x + u * blockDim.x;
                         if (vi < nvec) {
-                            u4* q = reinterpret_cast<u4*>(dst + (uint64_t)vi * 16);
-                            if (NT) __builtin_nontemporal_store(a[u], q);
-                            else *q = a[u];
+                            u4* q0 = reinterpret_cast<u4*>(d0 + (uint64_t)vi * 16);
+                            u4* q1 = reinterpret_cast<u4*>(d1 + (uint64_t)vi * 16);
+                            if (NT) {
+                                __builtin_nontemporal_store(a[u], q0);
+                                __builtin_nontemporal_store(c[u], q1);
+                            } else {
+                                *q0 = a[u];
+                                *q1 = c[u];
+                            }
                         }
                     }
                 }
             } else {
-                for (uint64_t b = threadIdx.x; b < (uint64_t)it.n * esize; b += blockDim.x) dst[b] = src[b];
+                const uint64_t nb = (uint64_t)it.n * esize;
+                for (uint64_t b = threadIdx.x; b < nb; b += blockDim.x) {
+                    const char x0 = s0[b], x1 = s1[b];
+                    d0[b] = x0;
+                    d1[b] = x1;
+                }
             }
         }
     }

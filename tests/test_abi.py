@@ -122,3 +122,30 @@ def test_remap_rank_export_against_golden_fixture():
         rows = json.load(f)["P_rank_remap"]
     for P, r, want in rows:  # including non-powers of two, where the decimal 80000000 test acts
         assert L.bine_remap_rank(r, P) == want, (P, r)
+
+
+def test_host_only_entry_points_check_their_arguments():
+    """Argument errors of the simulator, tuning and network entry points are
+    reported before any device call (so they run without a GPU)."""
+    L = _lib.load()
+    n = ctypes.c_uint32()
+    assert L.bine_sim_classes(None, ctypes.byref(n), None, None, None, None, None, None) == _lib.BINE_ERR_ARG
+    c = _lib.SimCase()
+    c.dimensions[0], c.dimensions_num, c.coll, c.algo = 8, 1, 0, 0
+    assert L.bine_sim_classes(ctypes.byref(c), ctypes.byref(n), None, None, None, None, None, None) == _lib.BINE_OK
+    assert n.value == 6  # 3 reduce-scatter + 3 allgather steps
+    dest = (ctypes.c_uint32 * 8)()
+    assert L.bine_sim_peers(ctypes.byref(c), 6, dest) == _lib.BINE_ERR_ARG  # no class 6
+    c.coll, c.algo = 3, 2  # bcast has no ring schedule
+    assert L.bine_sim_classes(ctypes.byref(c), ctypes.byref(n), None, None, None, None, None, None) == _lib.BINE_ERR_UNSUPPORTED
+    cnt = ctypes.c_int()
+    assert L.bine_tuning_candidates(0, None, 0, ctypes.byref(cnt)) == _lib.BINE_OK and cnt.value >= 20
+    assert L.bine_tuning_candidates(0, None, 4, ctypes.byref(cnt)) == _lib.BINE_ERR_ARG  # cap without a buffer
+    h = ctypes.c_void_p()
+    dims = (ctypes.c_int32 * 1)(18)
+    assert L.bine_mlp_create(dims, 1, 16, 0, 0, ctypes.byref(h)) == _lib.BINE_ERR_ARG  # a network needs 2+ widths
+    dims2 = (ctypes.c_int32 * 2)(18, 0)
+    assert L.bine_mlp_create(dims2, 2, 16, 0, 0, ctypes.byref(h)) == _lib.BINE_ERR_ARG
+    assert L.bine_mlp_forward(None, None, 1, None, None) == _lib.BINE_ERR_ARG
+    assert L.bine_sim_run(None, None) == _lib.BINE_ERR_ARG
+    assert L.bine_batch_time_shapes(None, None, 1, None, 1, None) == _lib.BINE_ERR_ARG

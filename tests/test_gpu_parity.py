@@ -75,7 +75,7 @@ def test_allreduce_small(gpu, dims, algo, dtype, exec):
         assert same_bits(host(r), ref)
 
 
-@pytest.mark.parametrize("ports", [2, 3])
+@pytest.mark.parametrize("ports", [2, 3, 6])  # 6 ports on 3 dimensions: three mirrored (lib/libbine_coll.cc:15-31)
 def test_allreduce_multiport(gpu, ports):
     x = per_rank(64, 64 * ports * 5 + 1, "float32", 3)
     ref = O.allreduce(x, [4, 4, 4], ports=ports)

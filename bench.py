@@ -142,7 +142,8 @@ def bucketed_p64(stream, seed=7, nb=256):
         batch = Batch(comm, [{"coll": "allreduce", "sendbuf": s, "recvbuf": r, "count": int(n)}
                              for s, r, n in zip(sends, recvs, sizes)])
         batch.autotune(iters=2, stream=stream)
-        b_wall, _ = time_launches(lambda: batch.launch_graph(stream), stream, 5, 2)
+        b_wall, _ = time_launches(lambda: batch.launch(stream), stream, 5, 2)
+        g_wall, _ = time_launches(lambda: batch.launch_graph(stream), stream, 5, 2)
         step = VComm([4, 4, 4], exec="STEP")
 
         def per_step():
@@ -153,7 +154,7 @@ def bucketed_p64(stream, seed=7, nb=256):
     same = all(torch.equal(a.view(torch.int32), c.view(torch.int32)) for a, c in zip(recvs, recvs2))
     out = {"workload": "bucketed_allreduce_p64_4x4x4_fp32_256_buckets", "buckets": nb,
            "bytes_per_rank": int(sizes.sum() * 4), "algorithmic_bytes": b_alg,
-           "batched_ms": round(b_wall, 3), "per_step_ms": round(s_wall, 3),
+           "batched_ms": round(b_wall, 3), "graph_replay_ms": round(g_wall, 3), "per_step_ms": round(s_wall, 3),
            "batched_GBps": round(b_alg / (b_wall * 1e-3) / 1e9, 1), "per_step_GBps": round(b_alg / (s_wall * 1e-3) / 1e9, 1),
            "batched_vs_per_step": round(s_wall / b_wall, 2), "per_step_launches": 13 * nb,
            "tuning": batch.tuning(), "bitwise_equal": same}
@@ -168,7 +169,8 @@ def mixed_8x8x8(stream, seed=21):
     alltoall int64 4 KiB per rank pair, bcast int32 64 KiB, scatter fp32 256 KiB
     blocks, gather int32 1 MiB blocks, reduce fp32 1 MiB; seeded uniform
     integers in [-1000, 1000] and N(0,1).  Batched: one fused work list per
-    element type, captured in one HIP graph.  Baseline: the six collectives as
+    element type, issued by bine_batch_launch (and replayed as a captured HIP
+    graph beside it).  Baseline: the six collectives as
     six single calls (each one fused launch; per-step execution of the rooted
     collectives at these sizes needs the reference's full-size temporary on
     every rank, P x 512 MiB for the gather)."""
@@ -199,7 +201,8 @@ def mixed_8x8x8(stream, seed=21):
         comm = VComm(DIMS, exec="FUSED")
         batch = Batch(comm, [{"coll": c, "sendbuf": s, "recvbuf": r, "count": n} for c, (s, r, n) in t.items()])
         batch.autotune(iters=2, stream=stream)
-        b_wall, _ = time_launches(lambda: batch.launch_graph(stream), stream, 10, 3)
+        b_wall, _ = time_launches(lambda: batch.launch(stream), stream, 10, 3)
+        g_wall, _ = time_launches(lambda: batch.launch_graph(stream), stream, 10, 3)
         outs = {c: r.clone() for c, (s, r, n) in t.items() if c != "bcast"}
         step = VComm(DIMS, exec="AUTO")
 
@@ -214,7 +217,8 @@ def mixed_8x8x8(stream, seed=21):
     torch.cuda.synchronize()
     same = all(torch.equal(outs[c].view(torch.uint8), t[c][1].view(torch.uint8)) for c in outs)
     out = {"workload": "mixed_8x8x8_allreduce_fp64_4MiB_alltoall_int64_4KiB_bcast_scatter_gather_reduce",
-           "algorithmic_bytes": int(b_alg), "batched_ms": round(b_wall, 3), "per_call_ms": round(s_wall, 3),
+           "algorithmic_bytes": int(b_alg), "batched_ms": round(b_wall, 3), "graph_replay_ms": round(g_wall, 3),
+           "per_call_ms": round(s_wall, 3),
            "batched_GBps": round(b_alg / (b_wall * 1e-3) / 1e9, 1), "per_call_GBps": round(b_alg / (s_wall * 1e-3) / 1e9, 1),
            "batched_vs_per_call": round(s_wall / b_wall, 2), "stats": batch.stats(), "tuning": batch.tuning(),
            "bitwise_equal": same}
@@ -256,9 +260,11 @@ def small_configs(stream, cpu=True):
             c = VComm(dims, exec="FUSED")
             bt = Batch(c, [{"coll": "allreduce", "sendbuf": x, "recvbuf": y, "count": count}])
             bt.autotune(iters=2, stream=stream)
-            w, _ = time_launches(lambda: bt.launch_graph(stream), stream, 20, 3)
+            w, _ = time_launches(lambda: bt.launch(stream), stream, 20, 3)
             res["batched_ms"] = round(w, 4)
             res["batched_GBps"] = round(b_alg / (w * 1e-3) / 1e9, 1)
+            w, _ = time_launches(lambda: bt.launch_graph(stream), stream, 20, 3)
+            res["graph_replay_ms"] = round(w, 4)
             bt.close()
             c.close()
         torch.cuda.synchronize()

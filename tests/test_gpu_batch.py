@@ -199,3 +199,34 @@ def test_harness_runs_and_validates(gpu, collective, type_name, count, capsys):
         assert run.validate()
     rec = H.format_record(samples).splitlines()
     assert rec[0] == "highest" and len(rec) == 5
+
+
+@pytest.mark.parametrize("coll", ["alltoall", "allgather", "scatter", "reduce_scatter"])
+def test_every_candidate_shape_gives_the_same_bits(gpu, coll):
+    """Copy runs and reductions at every autotuner candidate shape (vectors per
+    thread, workgroup, persistent grid, non-temporal) write identical bytes."""
+    from bine_trees_fugaku_amd.collectives import tuning_candidates
+    P, dims = 64, [4, 4, 4]
+    c = {"alltoall": 96, "allgather": 1000, "scatter": 3000, "reduce_scatter": 777}[coll]
+    if coll == "alltoall":
+        send, recv = randint((P, P * c), torch.int64, 1), torch.zeros((P, P * c), dtype=torch.int64, device="cuda")
+    elif coll == "allgather":
+        send, recv = randn((P, c), torch.float32, 2), torch.zeros((P, P * c), dtype=torch.float32, device="cuda")
+    elif coll == "scatter":
+        send, recv = randn((P * c,), torch.float64, 3), torch.zeros((P, c), dtype=torch.float64, device="cuda")
+    else:
+        send, recv = randn((P, P * c), torch.float32, 4), torch.zeros((P, c), dtype=torch.float32, device="cuda")
+    with VComm(dims, exec="FUSED") as comm:
+        b = Batch(comm, [{"coll": coll, "sendbuf": send, "recvbuf": recv, "count": c}])
+        b.launch()
+        torch.cuda.synchronize()
+        ref = recv.clone()
+        for vpt, thr, pers, nt in tuning_candidates():
+            recv.zero_()
+            b.set_tuning(int(vpt), int(thr), int(pers), int(nt))
+            b.launch()
+            torch.cuda.synchronize()
+            assert same_bits(recv, ref), (coll, vpt, thr, pers, nt)
+        b.close()
+    if coll == "alltoall":
+        assert np.array_equal(ref.cpu().numpy(), O.alltoall(send.cpu().numpy(), P))

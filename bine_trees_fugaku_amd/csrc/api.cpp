@@ -551,7 +551,7 @@ std::vector<LaunchShape> tuning_candidates(int cus) {
             {1, 256, 0, 1}, {2, 256, 0, 1}, {1, 512, 0, 1}, {2, 128, 0, 1},       {2, 64, 0, 1},
             {4, 64, 0, 1},  {4, 128, 0, 1}, {1, 64, 0, 1},  {4, 64, 8 * cus, 0},  {4, 64, 16 * cus, 1},
             {1, 256, 4 * cus, 0}, {2, 128, 8 * cus, 0}, {4, 256, 0, 1}, {4, 256, 0, 0}, {2, 256, 4 * cus, 1},
-            {4, 128, 8 * cus, 1}};
+            {4, 128, 8 * cus, 1}, {4, 512, 0, 1}, {4, 512, 4 * cus, 1}};
 }
 
 // Issue one call of a collective (argument layout of its entry point).

@@ -3,7 +3,7 @@ batch from its description, trained on measured kernel timings.
 
 The autotuner (bine_batch_autotune) times every candidate shape (vectors per
 thread, workgroup size, persistent grid, non-temporal access) on the real
-buffers: about 26 x (iters + 1) launches before the first useful one.  This
+buffers: about 28 x (iters + 1) launches before the first useful one.  This
 model predicts the time of every shape from a few features of the batch, so a
 new batch starts at a good shape without any timing run; the autotuner remains
 the fallback and the judge of the model's regret.

@@ -1655,6 +1655,7 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
         // output advance by one fixed (rank, offset) step become one ITEM_RUN item.
         int64_t run_item = -1;
         uint32_t run_len = 0;
+        std::vector<uint32_t> tile_no;  // per emitted item: its tile's index within its group
         PoolEntry run_leaf{}, run_out{}, step_leaf{}, step_out{};
         auto single_copy = [&](const Group& g) {
             return g.lg == 0 && g.leaves.size() == 1 && g.outs.size() == 1 && g.n <= shallow_tile && g.n % vec == 0 &&
@@ -1715,7 +1716,9 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
             bool same = true;
             for (const PoolEntry& e : g.leaves) same &= (e.off % vec) == res;
             for (const PoolEntry& e : g.outs) same &= (e.off % vec) == res;
+            uint32_t tiles = 0;
             auto emit = [&](int64_t d, int64_t n, bool v) {
+                tile_no.push_back(tiles++);
                 GredItem it{};
                 it.leaf_off = leaf_off;
                 it.out_off = out_off;
@@ -1749,11 +1752,26 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
             out->bytes_written += (uint64_t)g.n * g.outs.size() * dtsize;
             out->max_leaves = std::max<uint64_t>(out->max_leaves, nl);
         }
-        // run items go last: they are executed by their own kernel
-        auto mid = std::stable_partition(out->items.begin() + first, out->items.end(),
-                                         [](const GredItem& it) { return !(it.flags & ITEM_RUN); });
-        out->launches.push_back({first, (uint32_t)out->items.size() - first});
-        out->runs.push_back((uint32_t)(out->items.end() - mid));
+        // Order: tile 0 of every group, then tile 1 of every group, ...: workgroups that run at the same
+        // time then touch many blocks of every partition rather than neighbouring tiles of one block.
+        // Run items go last: they are executed by their own kernel.
+        const size_t cnt = out->items.size() - first;
+        std::vector<uint32_t> order(cnt);
+        for (size_t i = 0; i < cnt; ++i) order[i] = (uint32_t)i;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+            const bool ra = out->items[first + a].flags & ITEM_RUN, rb = out->items[first + b].flags & ITEM_RUN;
+            if (ra != rb) return rb;
+            return tile_no[a] < tile_no[b];
+        });
+        std::vector<GredItem> sorted(cnt);
+        uint32_t nrun = 0;
+        for (size_t i = 0; i < cnt; ++i) {
+            sorted[i] = out->items[first + order[i]];
+            nrun += (sorted[i].flags & ITEM_RUN) ? 1 : 0;
+        }
+        std::copy(sorted.begin(), sorted.end(), out->items.begin() + first);
+        out->launches.push_back({first, (uint32_t)cnt});
+        out->runs.push_back(nrun);
     }
 }
 

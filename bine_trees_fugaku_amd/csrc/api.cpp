@@ -778,35 +778,12 @@ int lower_list(bine_batch_s::TypeList& l) {
     {
         l.arr.release();
         const size_t ds = dtsize(l.dtype);
-        // lower each batch item's groups (a consecutive span with one bucket) together,
-        // so every work item carries its bucket index and runs of copies can form
+        // one lowering for the whole list: every work item carries its group's bucket index, runs of copies
+        // form within a bucket, and items are ordered tile-major across all buckets
         Lowered all;
-        std::vector<GredItem> run_items;  // appended after the others: they have a kernel of their own
-        int max_lg = 0;
-        const auto& gs = l.groups[0];
-        for (size_t g0 = 0; g0 < gs.size();) {
-            size_t g1 = g0 + 1;
-            while (g1 < gs.size() && l.group_bucket[g1] == l.group_bucket[g0]) ++g1;
-            Lowered one;
-            std::vector<std::vector<Group>> v(1, std::vector<Group>(gs.begin() + g0, gs.begin() + g1));
-            lower_groups(v, shallow_tile_elems(l.shape, ds), (int)(16 / ds), l.group_bucket[g0], ds, &one,
-                         deep_tile_elems(l.shape, ds), split_lg(l.shape));
-            const uint32_t base = (uint32_t)all.pool.size();
-            for (auto it : one.items) {
-                it.leaf_off += base;
-                it.out_off += base;
-                ((it.flags & ITEM_RUN) ? run_items : all.items).push_back(it);
-            }
-            all.pool.insert(all.pool.end(), one.pool.begin(), one.pool.end());
-            all.bytes_read += one.bytes_read;
-            all.bytes_written += one.bytes_written;
-            all.max_leaves = std::max(all.max_leaves, one.max_leaves);
-            for (size_t gi = g0; gi < g1; ++gi) max_lg = std::max<int>(max_lg, gs[gi].lg);
-            g0 = g1;
-        }
-        all.items.insert(all.items.end(), run_items.begin(), run_items.end());
-        all.launches.push_back({0, (uint32_t)all.items.size()});
-        all.runs.push_back((uint32_t)run_items.size());
+        lower_groups(l.groups, shallow_tile_elems(l.shape, ds), (int)(16 / ds), 0, ds, &all, deep_tile_elems(l.shape, ds),
+                     split_lg(l.shape), &l.group_bucket);
+        int max_lg = max_lg_of(l.groups);
         l.arr.low = std::move(all);
         l.arr.max_lg = max_lg;
         if (!l.arr.low.items.empty()) {

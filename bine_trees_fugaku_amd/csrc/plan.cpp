@@ -1648,7 +1648,8 @@ bool compile_program(const Program& prog, bool want_fused, CompiledPlan* out, st
 // ===========================================================================
 
 void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shallow_tile, int vec, uint32_t bucket,
-                  size_t dtsize, Lowered* out, int64_t deep_tile_elems, int deep_lg) {
+                  size_t dtsize, Lowered* out, int64_t deep_tile_elems, int deep_lg,
+                  const std::vector<uint32_t>* group_bucket) {
     for (const auto& gs : launches) {
         const uint32_t first = (uint32_t)out->items.size();
         // Open run of aligned single copies: consecutive groups whose leaf and
@@ -1670,9 +1671,12 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
         auto advance = [](const PoolEntry& a, const PoolEntry& st, uint32_t k) {
             return PoolEntry{a.rb + st.rb * k, a.off + st.off * k};
         };
-        for (const Group& g : gs) {
+        for (size_t gi = 0; gi < gs.size(); ++gi) {
+            const Group& g = gs[gi];
+            const uint32_t gb = group_bucket ? (*group_bucket)[gi] : bucket;
             if (g.n <= 0) continue;
-            if (single_copy(g) && run_item >= 0 && run_len < kMaxRun && out->items[run_item].n == (uint32_t)g.n) {
+            if (single_copy(g) && run_item >= 0 && run_len < kMaxRun && out->items[run_item].n == (uint32_t)g.n &&
+                out->items[run_item].bucket == gb) {
                 bool ok;
                 if (run_len == 1) {
                     PoolEntry sl, so;
@@ -1727,7 +1731,7 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
                 it.nouts = (uint32_t)g.outs.size();
                 it.delta = (uint32_t)d;
                 it.n = (uint32_t)n;
-                it.bucket = bucket;
+                it.bucket = gb;
                 out->items.push_back(it);
             };
             int64_t d = 0;

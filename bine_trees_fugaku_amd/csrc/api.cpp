@@ -262,9 +262,11 @@ BufTable make_table(const DevPlan* dp, const void* send, void* recv) {
     return t;
 }
 
+// 16-byte vector accesses need 4-byte aligned addresses only (global memory runs in the
+// unaligned access mode); partitions aligned below that (odd int8 counts) take the scalar path.
 uint32_t vec_ok(const BufTable& t) {
     for (int b = 0; b < 4; ++b)
-        if ((t.base[b] & 15) || (t.stride[b] & 15)) return 0;
+        if ((t.base[b] & 3) || (t.stride[b] & 3)) return 0;
     return 1;
 }
 
@@ -774,15 +776,17 @@ struct bine_batch_s {
 
 namespace {
 
-int lower_list(bine_batch_s::TypeList& l) {
+int lower_list(bine_batch_t b, bine_batch_s::TypeList& l) {
     {
         l.arr.release();
         const size_t ds = dtsize(l.dtype);
         // one lowering for the whole list: every work item carries its group's bucket index, runs of copies
         // form within a bucket, and items are ordered tile-major across all buckets
         Lowered all;
+        std::vector<uint8_t> bucket_vec(b->tabs.size());
+        for (size_t i = 0; i < b->tabs.size(); ++i) bucket_vec[i] = (uint8_t)vec_ok(b->tabs[i]);
         lower_groups(l.groups, shallow_tile_elems(l.shape, ds), (int)(16 / ds), 0, ds, &all, deep_tile_elems(l.shape, ds),
-                     split_lg(l.shape), &l.group_bucket);
+                     split_lg(l.shape), &l.group_bucket, &bucket_vec);
         int max_lg = max_lg_of(l.groups);
         l.arr.low = std::move(all);
         l.arr.max_lg = max_lg;
@@ -798,7 +802,7 @@ int lower_list(bine_batch_s::TypeList& l) {
 
 int batch_lower(bine_batch_t b) {
     for (auto& l : b->lists) {
-        int rc = lower_list(l);
+        int rc = lower_list(b, l);
         if (rc) return rc;
     }
     return BINE_OK;
@@ -972,7 +976,7 @@ extern "C" int bine_batch_set_tuning(bine_batch_t b, int32_t vpt, int32_t thread
         const bool relower = sh.vpt != l.shape.vpt || sh.threads != l.shape.threads;
         l.shape = sh;
         if (relower) {
-            int rc = lower_list(l);
+            int rc = lower_list(b, l);
             if (rc) return rc;
         }
     }
@@ -1061,7 +1065,7 @@ extern "C" int bine_batch_autotune(bine_batch_t b, void* stream, int iters) {
             auto& l = b->lists[i];
             const bool relower = c.vpt != l.shape.vpt || c.threads != l.shape.threads;
             l.shape = c;
-            int rc = relower ? lower_list(l) : BINE_OK;
+            int rc = relower ? lower_list(b, l) : BINE_OK;
             if (rc) return rc;
             if ((rc = run_list(b, l, st))) return rc;  // warm-up
             HIPCHK(hipEventRecord(ev.a, st));
@@ -1083,7 +1087,7 @@ extern "C" int bine_batch_autotune(bine_batch_t b, void* stream, int iters) {
         const bool relower = pick[i].vpt != l.shape.vpt || pick[i].threads != l.shape.threads;
         l.shape = pick[i];
         if (relower) {
-            int rc = lower_list(l);
+            int rc = lower_list(b, l);
             if (rc) return rc;
         }
         if (best[i] > best[dominant]) dominant = i;

@@ -1649,7 +1649,7 @@ bool compile_program(const Program& prog, bool want_fused, CompiledPlan* out, st
 
 void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shallow_tile, int vec, uint32_t bucket,
                   size_t dtsize, Lowered* out, int64_t deep_tile_elems, int deep_lg,
-                  const std::vector<uint32_t>* group_bucket) {
+                  const std::vector<uint32_t>* group_bucket, const std::vector<uint8_t>* bucket_vec) {
     for (const auto& gs : launches) {
         const uint32_t first = (uint32_t)out->items.size();
         // Open run of aligned single copies: consecutive groups whose leaf and
@@ -1658,9 +1658,10 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
         uint32_t run_len = 0;
         std::vector<uint32_t> tile_no;  // per emitted item: its tile's index within its group
         PoolEntry run_leaf{}, run_out{}, step_leaf{}, step_out{};
-        auto single_copy = [&](const Group& g) {
+        auto vec_bucket = [&](uint32_t b) { return !bucket_vec || (*bucket_vec)[b]; };
+        auto single_copy = [&](const Group& g, uint32_t b) {
             return g.lg == 0 && g.leaves.size() == 1 && g.outs.size() == 1 && g.n <= shallow_tile && g.n % vec == 0 &&
-                   g.leaves[0].off % vec == 0 && g.outs[0].off % vec == 0;
+                   g.leaves[0].off % vec == 0 && g.outs[0].off % vec == 0 && vec_bucket(b);
         };
         // step b - a with non-negative rank and offset parts in the same buffer
         auto step_of = [](const PoolEntry& a, const PoolEntry& b, PoolEntry* st) {
@@ -1675,7 +1676,7 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
             const Group& g = gs[gi];
             const uint32_t gb = group_bucket ? (*group_bucket)[gi] : bucket;
             if (g.n <= 0) continue;
-            if (single_copy(g) && run_item >= 0 && run_len < kMaxRun && out->items[run_item].n == (uint32_t)g.n &&
+            if (single_copy(g, gb) && run_item >= 0 && run_len < kMaxRun && out->items[run_item].n == (uint32_t)g.n &&
                 out->items[run_item].bucket == gb) {
                 bool ok;
                 if (run_len == 1) {
@@ -1717,7 +1718,7 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
             out->pool.insert(out->pool.end(), g.outs.begin(), g.outs.end());
             // all addresses share one residue modulo the vector width?
             int64_t res = g.leaves[0].off % vec;
-            bool same = true;
+            bool same = vec_bucket(gb);
             for (const PoolEntry& e : g.leaves) same &= (e.off % vec) == res;
             for (const PoolEntry& e : g.outs) same &= (e.off % vec) == res;
             uint32_t tiles = 0;
@@ -1745,7 +1746,7 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
                 d += body;
             }
             for (; d < g.n; d += tile_elems) emit(d, std::min(tile_elems, g.n - d), false);
-            if (single_copy(g)) {  // exactly one aligned item: a run may start here
+            if (single_copy(g, gb)) {  // exactly one aligned item: a run may start here
                 run_item = (int64_t)out->items.size() - 1;
                 run_len = 1;
                 run_leaf = g.leaves[0];

@@ -113,9 +113,10 @@ struct Lowered {
 // Groups whose tree has at least 2^deep_lg leaves use deep_tile_elems instead:
 // the kernel splits their leaves across the waves of a workgroup, so their
 // tile is one wave wide.
-// group_bucket (one launch only): the buffer-table index of every group, instead of `bucket`.
+// group_bucket (one launch only): the buffer-table index of every group, instead of `bucket`;
+// bucket_vec: per buffer table, whether its addresses allow 16-byte vector access (no ITEM_VEC otherwise).
 void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t tile_elems, int vec_elems,
                   uint32_t bucket, size_t dtsize, Lowered* out, int64_t deep_tile_elems = 0, int deep_lg = 99,
-                  const std::vector<uint32_t>* group_bucket = nullptr);
+                  const std::vector<uint32_t>* group_bucket = nullptr, const std::vector<uint8_t>* bucket_vec = nullptr);
 
 }  // namespace bine

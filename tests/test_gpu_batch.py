@@ -25,7 +25,8 @@ def randn(shape, dtype, seed):
 def randint(shape, dtype, seed):
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
-    return torch.randint(-1000, 1001, shape, dtype=dtype, device="cuda", generator=g)
+    lo, hi = (-128, 128) if dtype == torch.int8 else (-1000, 1001)
+    return torch.randint(lo, hi, shape, dtype=dtype, device="cuda", generator=g)
 
 
 def same_bits(a, b):
@@ -165,6 +166,29 @@ def test_mixed_collectives_8x8x8(gpu, scale):
         assert np.array_equal(got["scatter"].cpu().numpy(), O.scatter(a["scatter"][0], P, 0))
         assert np.array_equal(got["gather"].cpu().numpy().reshape(-1), O.gather(a["gather"][0], 0).reshape(-1))
         assert np.array_equal(got["reduce"].cpu().numpy().reshape(-1), O.reduce(a["reduce"][0], 0).reshape(-1))
+
+
+def test_batch_alignment_per_bucket(gpu):
+    """One batch mixing int8 buckets whose strides are not 4-byte aligned (scalar items) with int8, int32 and
+    float32 buckets whose strides are 4-byte but not 16-byte aligned (vector items): every bucket against the
+    oracle."""
+    P, dims = 16, [4, 4]
+    specs = [(torch.int8, 16 * 9 + 1), (torch.int8, 16 * 9 + 4), (torch.int32, 16 * 33 + 5),
+             (torch.float32, 16 * 21 + 3), (torch.int8, 16 * 64 + 7), (torch.float32, 16 * 64)]
+    sends = [(randint if dt != torch.float32 else randn)((P, n), dt, 40 + i) for i, (dt, n) in enumerate(specs)]
+    recvs = [torch.zeros_like(s) for s in sends]
+    with VComm(dims, exec="FUSED") as comm:
+        b = Batch(comm, [{"coll": "allreduce", "sendbuf": s, "recvbuf": r, "count": s.shape[1]}
+                         for s, r in zip(sends, recvs)])
+        for replay in range(2):
+            for r in recvs:
+                r.zero_()
+            b.launch() if replay == 0 else b.launch_graph()
+            torch.cuda.synchronize()
+            for i, (s, r) in enumerate(zip(sends, recvs)):
+                want = O.allreduce(s.cpu().numpy(), dims)
+                assert r.cpu().numpy().tobytes() == want.tobytes(), (replay, i, specs[i])
+        b.close()
 
 
 def test_batch_rejects_bad_items(gpu):

@@ -18,7 +18,7 @@ from oracle import oracle as O  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
-NP = {"int32": np.int32, "int64": np.int64, "float32": np.float32, "float64": np.float64}
+NP = {"int8": np.int8, "int32": np.int32, "int64": np.int64, "float32": np.float32, "float64": np.float64}
 
 
 def data(shape, dtype, seed):
@@ -71,6 +71,22 @@ def test_allreduce_small(gpu, dims, algo, dtype, exec):
     ref = O.allreduce(x, dims, algo=algo)
     with VComm(dims, exec=exec, allreduce_algo=algo) as c:
         r = torch.zeros((P, count), dtype=getattr(torch, dtype), device="cuda")
+        c.allreduce(dev(x), r)
+        assert same_bits(host(r), ref)
+
+
+@pytest.mark.parametrize("count_per_rank", [37, 40])  # int8 strides 37P+3 (not 4-aligned: scalar) and 40P+4
+@pytest.mark.parametrize("exec", ["STEP", "FUSED"])
+def test_allreduce_int8_alignment(gpu, count_per_rank, exec):
+    """int8 partitions whose stride is not a multiple of 4 bytes take the scalar path; a 4-byte stride keeps the
+    16-byte vector path.  Both wrap like the reference's char sum."""
+    dims = [4, 4]
+    P = 16
+    count = count_per_rank * P + (3 if count_per_rank % 2 else 4)
+    x = per_rank(P, count, "int8", 13)
+    ref = O.allreduce(x, dims)
+    with VComm(dims, exec=exec) as c:
+        r = torch.zeros((P, count), dtype=torch.int8, device="cuda")
         c.allreduce(dev(x), r)
         assert same_bits(host(r), ref)
 

@@ -33,6 +33,7 @@ SOURCES = [
     ("api.cpp", "hiphost"),
     ("sim.cpp", "hiphost"),
     ("gred.hip", "hip"),
+    ("gred_realign.hip", "hip"),
     ("sim.hip", "hip"),
     ("mlp.hip", "hip"),
 ]

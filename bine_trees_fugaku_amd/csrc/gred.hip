@@ -18,6 +18,10 @@
 
 #include "gred.hpp"
 
+#ifndef BINE_GRED_REALIGN
+#define BINE_GRED_REALIGN 0  // gred_realign.hip compiles this file again with 1
+#endif
+
 namespace bine {
 
 typedef unsigned int u4 __attribute__((ext_vector_type(4)));
@@ -81,6 +85,99 @@ template <typename T, bool NT>
 __device__ __forceinline__ void vstore(char* p, const Vec<T>& v) {
     if (NT) __builtin_nontemporal_store(v.raw, reinterpret_cast<u4*>(p));
     else *reinterpret_cast<u4*>(p) = v.raw;
+}
+
+// ---- realigned stores ----------------------------------------------------------
+// A vector tile's output can start 4, 8 or 12 bytes past a 16-byte boundary
+// (partition strides that are not multiples of 16 bytes, e.g. odd fp32
+// counts, or an output block whose offset has another residue than its
+// source's).  gfx950 serves unaligned 16-byte loads at full rate but unaligned
+// 16-byte stores about 15 % slower than aligned ones, so the realigning
+// gather-reduce kernels (REALIGN = true; the host launches them for
+// store-heavy work lists, where stores dominate, that can have unaligned
+// outputs) write such outputs as aligned chunks: with the output k dwords past a boundary,
+// chunk c holds the last k dwords of vector c - 1 and the first 4 - k of
+// vector c.  Each wave holds 64 * VPT consecutive vectors (lane l of round u
+// holds vector 64u + l); a lane takes its predecessor's last three words by a
+// lane rotation (lane 0 of round u from lane 63 of round u - 1), the wave's
+// first vector writes only its own part of its chunk and the wave's last
+// vector also writes its last k dwords into the following chunk.  Every byte
+// of the output range is written exactly once.
+typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+
+struct Prev {
+    unsigned y, z, w;
+};
+
+// lane l receives the words of lane (l - 1) mod 64
+__device__ __forceinline__ Prev rotate_prev(const u4& v, uint32_t lane) {
+    const int src = (int)((lane + 63u) & 63u);
+    return Prev{(unsigned)__shfl((int)v.y, src, 64), (unsigned)__shfl((int)v.z, src, 64),
+                (unsigned)__shfl((int)v.w, src, 64)};
+}
+
+template <bool NT>
+__device__ __forceinline__ void st_word(char* p, unsigned v) {
+    if (NT) __builtin_nontemporal_store(v, reinterpret_cast<unsigned*>(p));
+    else *reinterpret_cast<unsigned*>(p) = v;
+}
+
+template <bool NT>
+__device__ __forceinline__ void st_pair(char* p, const u2& v) {  // p 8-byte aligned
+    if (NT) __builtin_nontemporal_store(v, reinterpret_cast<u2*>(p));
+    else *reinterpret_cast<u2*>(p) = v;
+}
+
+template <bool NT>
+__device__ __forceinline__ void st_quad(char* p, const u4& v) {
+    if (NT) __builtin_nontemporal_store(v, reinterpret_cast<u4*>(p));
+    else *reinterpret_cast<u4*>(p) = v;
+}
+
+// p: this lane's vector address, k = (p mod 16) / 4 in 1..3 (the same for the
+// whole wave); pv: the previous vector's last three words; first: the previous
+// vector is not this wave's; last: the next vector is not this wave's.
+template <bool NT>
+__device__ __forceinline__ void store_shifted(char* p, uint32_t k, const u4& o, const Prev& pv, bool first, bool last) {
+    char* c = p - 4 * k;  // the aligned chunk that holds the vector's first 4 - k dwords
+    if (!first) {
+        const u4 v = k == 1 ? u4{pv.w, o.x, o.y, o.z} : k == 2 ? u4{pv.z, pv.w, o.x, o.y} : u4{pv.y, pv.z, pv.w, o.x};
+        st_quad<NT>(c, v);
+    } else if (k == 1) {  // own dwords 0..2 at chunk dwords 1..3
+        st_word<NT>(p, o.x);
+        st_pair<NT>(p + 4, u2{o.y, o.z});
+    } else if (k == 2) {
+        st_pair<NT>(p, u2{o.x, o.y});
+    } else {
+        st_word<NT>(p, o.x);
+    }
+    if (last) {  // own dwords 4-k..3 at the next chunk's dwords 0..k-1
+        char* n = c + 16;
+        if (k == 1) {
+            st_word<NT>(n, o.w);
+        } else if (k == 2) {
+            st_pair<NT>(n, u2{o.z, o.w});
+        } else {
+            st_pair<NT>(n, u2{o.y, o.z});
+            st_word<NT>(n + 8, o.w);
+        }
+    }
+}
+
+// One wave's 64 * VPT consecutive vectors (round u, lane l: vector index
+// vi[u]) of one output p, realigned; rot[u] = rotate_prev of round u.
+template <int VPT, bool NT>
+__device__ __forceinline__ void store_rounds_shifted(char* p, uint32_t k, const u4 (&v)[VPT], const Prev (&rot)[VPT],
+                                                     const uint32_t (&vi)[VPT], const bool (&ok)[VPT], uint32_t lane,
+                                                     uint32_t nvec) {
+#pragma unroll
+    for (int u = 0; u < VPT; ++u) {
+        if (ok[u]) {
+            const Prev pv = (u > 0 && lane == 0) ? rot[u > 0 ? u - 1 : 0] : rot[u];
+            store_shifted<NT>(p + (uint64_t)vi[u] * 16, k, v[u], pv, u == 0 && lane == 0,
+                              vi[u] + 1 == nvec || (u == VPT - 1 && lane == 63u));
+        }
+    }
 }
 
 // Buffer address of a (buffer, rank, element offset) entry.  The buffer is
@@ -156,7 +253,7 @@ __global__ void __launch_bounds__(512) grun_kernel(const GredItem* __restrict__ 
         const PoolEntry l0 = pool[it.leaf_off], ls = pool[it.leaf_off + 1];
         const PoolEntry o0 = pool[it.out_off], os = pool[it.out_off + 1];
         const uint64_t delta = it.delta;
-        const bool vec = (it.flags & ITEM_VEC) && allow_vec;
+        const bool vec = (it.flags & ITEM_VEC) && (allow_vec & 1u);
         // two copies per pass, so every thread keeps 2 * VPT 16-byte loads in flight
         for (uint32_t j = 0; j < it.pad; j += 2) {
             const uint32_t j1 = j + 1 < it.pad ? j + 1 : j;  // an odd tail copies its last block twice
@@ -211,7 +308,7 @@ struct Pack {
     Vec<T> x[VPT];
 };
 
-template <typename T, int VPT, int MAXS, bool BATCH, bool NT>
+template <typename T, int VPT, int MAXS, bool BATCH, bool NT, bool REALIGN>
 __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ items, uint32_t nitems,
                                                    const PoolEntry* __restrict__ pool, BufTable tab,
                                                    const BufTable* __restrict__ tabs, uint32_t allow_vec) {
@@ -241,7 +338,7 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
             for (int u = 0; u < VPT; ++u) r.x[u] = vadd<T>(a.x[u], b.x[u]);
             return r;
         };
-        const bool vec = (it.flags & ITEM_VEC) && allow_vec;
+        const bool vec = (it.flags & ITEM_VEC) && (allow_vec & 1u);
         if (vec && nw > 1 && lg >= lgw + 3) {
             // Deep tree: wave w reduces the contiguous leaf range [w << lgs, (w+1) << lgs),
             // a whole subtree of the balanced tree; the top lgw levels are then
@@ -287,12 +384,34 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
 #pragma unroll
                 for (int u = 0; u < VPT; ++u) sum.x[u] = lds[(size_t)u * 64 + lane];
                 __syncthreads();  // the next pass reuses the LDS slots
-                for (uint32_t o = wave; o < it.nouts; o += nw) {
-                    const PoolEntry e = outs[o];
-                    char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
+                if (REALIGN) {
+                    u4 v[VPT];
+                    Prev rot[VPT];
 #pragma unroll
-                    for (int u = 0; u < VPT; ++u)
-                        if (ok[u]) vstore<T, NT>(p + (uint64_t)vi[u] * 16, sum.x[u]);
+                    for (int u = 0; u < VPT; ++u) {
+                        v[u] = sum.x[u].raw;
+                        rot[u] = rotate_prev(v[u], lane);
+                    }
+                    for (uint32_t o = wave; o < it.nouts; o += nw) {
+                        const PoolEntry e = outs[o];
+                        char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
+                        const uint32_t m = (uint32_t)(uintptr_t)p & 15u;
+                        if (m != 0 && (m & 3u) == 0) {
+                            store_rounds_shifted<VPT, NT>(p, m >> 2, v, rot, vi, ok, lane, nvec);
+                        } else {
+#pragma unroll
+                            for (int u = 0; u < VPT; ++u)
+                                if (ok[u]) vstore<T, NT>(p + (uint64_t)vi[u] * 16, sum.x[u]);
+                        }
+                    }
+                } else {
+                    for (uint32_t o = wave; o < it.nouts; o += nw) {
+                        const PoolEntry e = outs[o];
+                        char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
+#pragma unroll
+                        for (int u = 0; u < VPT; ++u)
+                            if (ok[u]) vstore<T, NT>(p + (uint64_t)vi[u] * 16, sum.x[u]);
+                    }
                 }
             }
         } else if (vec) {
@@ -302,7 +421,8 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
                 bool ok[VPT];
 #pragma unroll
                 for (int u = 0; u < VPT; ++u) {
-                    vi[u] = base + threadIdx.x + u * blockDim.x;
+                    // realigning: each wave holds 64 * VPT consecutive vectors
+                    vi[u] = REALIGN ? base + wave * (64u * VPT) + u * 64u + lane : base + threadIdx.x + u * blockDim.x;
                     ok[u] = vi[u] < nvec;
                 }
                 auto loadp = [&](uint32_t k) __attribute__((always_inline)) {
@@ -317,12 +437,34 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
                     return r;
                 };
                 Pack<T, VPT> s = tree_eval<Pack<T, VPT>, MAXS, (VPT == 4 ? 4 : 8)>(lg, addp, loadp);
-                for (uint32_t o = 0; o < it.nouts; ++o) {
-                    const PoolEntry e = outs[o];
-                    char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
+                if (REALIGN) {
+                    u4 v[VPT];
+                    Prev rot[VPT];
 #pragma unroll
-                    for (int u = 0; u < VPT; ++u)
-                        if (ok[u]) vstore<T, NT>(p + (uint64_t)vi[u] * 16, s.x[u]);
+                    for (int u = 0; u < VPT; ++u) {
+                        v[u] = s.x[u].raw;
+                        rot[u] = rotate_prev(v[u], lane);
+                    }
+                    for (uint32_t o = 0; o < it.nouts; ++o) {
+                        const PoolEntry e = outs[o];
+                        char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
+                        const uint32_t m = (uint32_t)(uintptr_t)p & 15u;
+                        if (m != 0 && (m & 3u) == 0) {
+                            store_rounds_shifted<VPT, NT>(p, m >> 2, v, rot, vi, ok, lane, nvec);
+                        } else {
+#pragma unroll
+                            for (int u = 0; u < VPT; ++u)
+                                if (ok[u]) vstore<T, NT>(p + (uint64_t)vi[u] * 16, s.x[u]);
+                        }
+                    }
+                } else {
+                    for (uint32_t o = 0; o < it.nouts; ++o) {
+                        const PoolEntry e = outs[o];
+                        char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
+#pragma unroll
+                        for (int u = 0; u < VPT; ++u)
+                            if (ok[u]) vstore<T, NT>(p + (uint64_t)vi[u] * 16, s.x[u]);
+                    }
                 }
             }
         } else {
@@ -350,7 +492,7 @@ __global__ void reduce_local_kernel(const T* __restrict__ in, T* __restrict__ in
 
 // ---- host-side dispatch ------------------------------------------------------
 
-template <typename T, int VPT, int MAXS, bool BATCH>
+template <typename T, int VPT, int MAXS, bool BATCH, bool RA>
 static hipError_t launch_t(const GredItem* items, uint32_t n, const PoolEntry* pool, const BufTable& tab,
                            const BufTable* tabs, uint32_t allow_vec, const LaunchShape& sh, hipStream_t st) {
     uint32_t grid = sh.persistent > 0 ? (uint32_t)sh.persistent : n;
@@ -359,39 +501,39 @@ static hipError_t launch_t(const GredItem* items, uint32_t n, const PoolEntry* p
     if (sh.threads < 64 || sh.threads > 512 || (sh.threads & (sh.threads - 1))) return hipErrorInvalidValue;
     const size_t lds = (size_t)(sh.threads / 64) * VPT * 64 * 16;
     if (sh.nontemporal)
-        hipLaunchKernelGGL((gred_kernel<T, VPT, MAXS, BATCH, true>), dim3(grid), dim3(sh.threads), lds, st, items, n,
-                           pool, tab, tabs, allow_vec);
+        hipLaunchKernelGGL((gred_kernel<T, VPT, MAXS, BATCH, true, RA>), dim3(grid), dim3(sh.threads), lds, st, items,
+                           n, pool, tab, tabs, allow_vec);
     else
-        hipLaunchKernelGGL((gred_kernel<T, VPT, MAXS, BATCH, false>), dim3(grid), dim3(sh.threads), lds, st, items, n,
-                           pool, tab, tabs, allow_vec);
+        hipLaunchKernelGGL((gred_kernel<T, VPT, MAXS, BATCH, false, RA>), dim3(grid), dim3(sh.threads), lds, st, items,
+                           n, pool, tab, tabs, allow_vec);
     return hipGetLastError();
 }
 
-template <typename T, bool BATCH>
+template <typename T, bool BATCH, bool RA>
 static hipError_t launch_v(int max_lg, const GredItem* items, uint32_t n, const PoolEntry* pool, const BufTable& tab,
                            const BufTable* tabs, uint32_t allow_vec, const LaunchShape& sh, hipStream_t st) {
     const bool small = max_lg <= 9;
     switch (sh.vpt) {
     case 1:
-        return small ? launch_t<T, 1, 6, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st)
-                     : launch_t<T, 1, 11, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st);
+        return small ? launch_t<T, 1, 6, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st)
+                     : launch_t<T, 1, 11, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st);
     case 2:
-        return small ? launch_t<T, 2, 6, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st)
-                     : launch_t<T, 2, 11, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st);
+        return small ? launch_t<T, 2, 6, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st)
+                     : launch_t<T, 2, 11, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st);
     default:  // four vectors per thread: groups of 4 leaves, one more stack level; deep trees run with two
-        return small ? launch_t<T, 4, 7, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st)
-                     : launch_t<T, 2, 11, BATCH>(items, n, pool, tab, tabs, allow_vec, sh, st);
+        return small ? launch_t<T, 4, 7, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st)
+                     : launch_t<T, 2, 11, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st);
     }
 }
 
 // dtype codes follow bine_datatype_t
-hipError_t launch_gred(int dtype, int max_lg, const GredItem* items, uint32_t n, const PoolEntry* pool,
-                       const BufTable& tab, const BufTable* tabs, bool batch, uint32_t allow_vec,
-                       const LaunchShape& sh, hipStream_t st) {
-    if (n == 0) return hipSuccess;
+template <bool RA>
+hipError_t launch_gred_t(int dtype, int max_lg, const GredItem* items, uint32_t n, const PoolEntry* pool,
+                         const BufTable& tab, const BufTable* tabs, bool batch, uint32_t allow_vec,
+                         const LaunchShape& sh, hipStream_t st) {
 #define BINE_DISPATCH(T)                                                                               \
-    return batch ? launch_v<T, true>(max_lg, items, n, pool, tab, tabs, allow_vec, sh, st)             \
-                 : launch_v<T, false>(max_lg, items, n, pool, tab, tabs, allow_vec, sh, st)
+    return batch ? launch_v<T, true, RA>(max_lg, items, n, pool, tab, tabs, allow_vec, sh, st)         \
+                 : launch_v<T, false, RA>(max_lg, items, n, pool, tab, tabs, allow_vec, sh, st)
     switch (dtype) {
     case 0: BINE_DISPATCH(int32_t);
     case 1: BINE_DISPATCH(int64_t);
@@ -401,6 +543,23 @@ hipError_t launch_gred(int dtype, int max_lg, const GredItem* items, uint32_t n,
     default: return hipErrorInvalidValue;
     }
 #undef BINE_DISPATCH
+}
+
+#if BINE_GRED_REALIGN
+// this unit holds the realigning instantiations only
+template hipError_t launch_gred_t<true>(int, int, const GredItem*, uint32_t, const PoolEntry*, const BufTable&,
+                                        const BufTable*, bool, uint32_t, const LaunchShape&, hipStream_t);
+#else
+extern template hipError_t launch_gred_t<true>(int, int, const GredItem*, uint32_t, const PoolEntry*, const BufTable&,
+                                               const BufTable*, bool, uint32_t, const LaunchShape&, hipStream_t);
+
+// allow_vec: bit 0 permits ITEM_VEC items; bit 1 selects the realigning kernels
+hipError_t launch_gred(int dtype, int max_lg, const GredItem* items, uint32_t n, const PoolEntry* pool,
+                       const BufTable& tab, const BufTable* tabs, bool batch, uint32_t allow_vec,
+                       const LaunchShape& sh, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    return (allow_vec & 2u) ? launch_gred_t<true>(dtype, max_lg, items, n, pool, tab, tabs, batch, allow_vec, sh, st)
+                            : launch_gred_t<false>(dtype, max_lg, items, n, pool, tab, tabs, batch, allow_vec, sh, st);
 }
 
 hipError_t launch_grun(uint32_t esize, const GredItem* items, uint32_t n, const PoolEntry* pool, const BufTable& tab,
@@ -442,5 +601,7 @@ hipError_t launch_reduce_local(int dtype, const void* in, void* inout, uint64_t 
     }
     return hipGetLastError();
 }
+
+#endif  // BINE_GRED_REALIGN
 
 }  // namespace bine

@@ -1659,9 +1659,12 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
         std::vector<uint32_t> tile_no;  // per emitted item: its tile's index within its group
         PoolEntry run_leaf{}, run_out{}, step_leaf{}, step_out{};
         auto vec_bucket = [&](uint32_t b) { return !bucket_vec || (*bucket_vec)[b]; };
+        // 16-byte vector accesses need dword-aligned addresses only: loads run at full rate unaligned and
+        // the kernels realign the stores, so leaves and outputs need not share a residue modulo the vector.
+        auto word_ok = [&](const PoolEntry& e) { return ((uint64_t)e.off * dtsize) % 4 == 0; };
         auto single_copy = [&](const Group& g, uint32_t b) {
             return g.lg == 0 && g.leaves.size() == 1 && g.outs.size() == 1 && g.n <= shallow_tile && g.n % vec == 0 &&
-                   g.leaves[0].off % vec == 0 && g.outs[0].off % vec == 0 && vec_bucket(b);
+                   word_ok(g.leaves[0]) && word_ok(g.outs[0]) && vec_bucket(b);
         };
         // step b - a with non-negative rank and offset parts in the same buffer
         auto step_of = [](const PoolEntry& a, const PoolEntry& b, PoolEntry* st) {
@@ -1716,11 +1719,12 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
             out->pool.insert(out->pool.end(), g.leaves.begin(), g.leaves.end());
             const uint32_t out_off = (uint32_t)out->pool.size();
             out->pool.insert(out->pool.end(), g.outs.begin(), g.outs.end());
-            // all addresses share one residue modulo the vector width?
-            int64_t res = g.leaves[0].off % vec;
+            // vectors need every address dword-aligned; the body starts where the first output's offset is
+            // a whole number of vectors (outputs sharing its residue then store without realignment)
+            const int64_t res = (g.outs.empty() ? g.leaves[0].off : g.outs[0].off) % vec;
             bool same = vec_bucket(gb);
-            for (const PoolEntry& e : g.leaves) same &= (e.off % vec) == res;
-            for (const PoolEntry& e : g.outs) same &= (e.off % vec) == res;
+            for (const PoolEntry& e : g.leaves) same &= word_ok(e);
+            for (const PoolEntry& e : g.outs) same &= word_ok(e);
             uint32_t tiles = 0;
             auto emit = [&](int64_t d, int64_t n, bool v) {
                 tile_no.push_back(tiles++);
@@ -1779,6 +1783,15 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
             nrun += (sorted[i].flags & ITEM_RUN) ? 1 : 0;
         }
         std::copy(sorted.begin(), sorted.end(), out->items.begin() + first);
+        for (const GredItem& it : sorted) {
+            if (!(it.flags & ITEM_VEC)) continue;
+            if (it.flags & ITEM_RUN) {  // first output and the per-copy step
+                out->unaligned_outs |= out->pool[it.out_off].off % vec != 0 || out->pool[it.out_off + 1].off % vec != 0;
+            } else {
+                for (uint32_t o = 0; o < it.nouts; ++o)
+                    out->unaligned_outs |= ((int64_t)out->pool[it.out_off + o].off + it.delta) % vec != 0;
+            }
+        }
         out->launches.push_back({first, (uint32_t)cnt});
         out->runs.push_back(nrun);
     }

@@ -108,6 +108,7 @@ struct Lowered {
     std::vector<std::pair<uint32_t, uint32_t>> launches;  // (first item, item count)
     std::vector<uint32_t> runs;  // per launch: how many of its items are ITEM_RUN items (they come last)
     uint64_t bytes_read = 0, bytes_written = 0, max_leaves = 0;
+    bool unaligned_outs = false;  // some vector item writes an output whose offset is not a whole number of vectors
 };
 // tile_elems: elements per work item; vec_elems: elements per 16-byte vector.
 // Groups whose tree has at least 2^deep_lg leaves use deep_tile_elems instead:

@@ -225,32 +225,50 @@ def test_harness_runs_and_validates(gpu, collective, type_name, count, capsys):
     assert rec[0] == "highest" and len(rec) == 5
 
 
-@pytest.mark.parametrize("coll", ["alltoall", "allgather", "scatter", "reduce_scatter"])
+@pytest.mark.parametrize("coll", ["alltoall", "allgather", "scatter", "reduce_scatter", "allreduce_f32",
+                                  "allreduce_f64", "allreduce_i8", "bcast"])
 def test_every_candidate_shape_gives_the_same_bits(gpu, coll):
     """Copy runs and reductions at every autotuner candidate shape (vectors per
-    thread, workgroup, persistent grid, non-temporal) write identical bytes."""
+    thread, workgroup, persistent grid, non-temporal) write the oracle's bytes.
+    The odd counts put partitions and blocks 4, 8 or 12 bytes past 16-byte
+    boundaries: the store-heavy allgather and bcast lists run the realigning
+    kernels (aligned chunks built with lane shuffles), the others store
+    unaligned 16-byte vectors."""
     from bine_trees_fugaku_amd.collectives import tuning_candidates
     P, dims = 64, [4, 4, 4]
-    c = {"alltoall": 96, "allgather": 1000, "scatter": 3000, "reduce_scatter": 777}[coll]
+    c = {"alltoall": 96, "allgather": 1001, "bcast": 64 * 55 + 7, "scatter": 3001, "reduce_scatter": 777, "allreduce_f32": 64 * 37 + 5,
+         "allreduce_f64": 64 * 29 + 3, "allreduce_i8": 64 * 80 + 12}[coll]
     if coll == "alltoall":
         send, recv = randint((P, P * c), torch.int64, 1), torch.zeros((P, P * c), dtype=torch.int64, device="cuda")
+        want = O.alltoall(send.cpu().numpy(), P)
     elif coll == "allgather":
         send, recv = randn((P, c), torch.float32, 2), torch.zeros((P, P * c), dtype=torch.float32, device="cuda")
+        want = O.allgather(send.cpu().numpy(), dims)
     elif coll == "scatter":
         send, recv = randn((P * c,), torch.float64, 3), torch.zeros((P, c), dtype=torch.float64, device="cuda")
-    else:
+        want = O.scatter(send.cpu().numpy(), P)
+    elif coll == "bcast":
+        send, recv = None, torch.zeros((P, c), dtype=torch.float32, device="cuda")
+        root_row = randn((c,), torch.float32, 6)
+        want = np.tile(root_row.cpu().numpy().reshape(1, -1), (P, 1))
+    elif coll == "reduce_scatter":
         send, recv = randn((P, P * c), torch.float32, 4), torch.zeros((P, c), dtype=torch.float32, device="cuda")
+        want = O.reduce_scatter(send.cpu().numpy(), dims)
+    else:
+        dt = {"allreduce_f32": torch.float32, "allreduce_f64": torch.float64, "allreduce_i8": torch.int8}[coll]
+        send = (randint if dt == torch.int8 else randn)((P, c), dt, 5)
+        recv = torch.zeros_like(send)
+        want = O.allreduce(send.cpu().numpy(), dims)
     with VComm(dims, exec="FUSED") as comm:
-        b = Batch(comm, [{"coll": coll, "sendbuf": send, "recvbuf": recv, "count": c}])
-        b.launch()
-        torch.cuda.synchronize()
-        ref = recv.clone()
+        b = Batch(comm, [{"coll": coll.split("_")[0] if coll.startswith("allreduce") else coll,
+                          "sendbuf": send, "recvbuf": recv, "count": c}])
         for vpt, thr, pers, nt in tuning_candidates():
             recv.zero_()
+            if coll == "bcast":
+                recv[0] = root_row
             b.set_tuning(int(vpt), int(thr), int(pers), int(nt))
             b.launch()
             torch.cuda.synchronize()
-            assert same_bits(recv, ref), (coll, vpt, thr, pers, nt)
+            got = recv.cpu().numpy()
+            assert got.tobytes() == want.reshape(got.shape).tobytes(), (coll, vpt, thr, pers, nt)
         b.close()
-    if coll == "alltoall":
-        assert np.array_equal(ref.cpu().numpy(), O.alltoall(send.cpu().numpy(), P))

@@ -230,3 +230,32 @@ def test_fused_bytes_are_compulsory():
     writes = sum(int(it["n"]) * int(it["nouts"]) for it in items)
     assert reads == P * count and writes == P * count
     assert len(launches) == 1
+
+
+@pytest.mark.parametrize("coll,dims,count,dtype", [
+    ("allgather", [4, 4, 4], 1001, "float32"), ("allreduce", [4, 4], 16 * 37 + 3, "float32"),
+    ("alltoall", [8], 17, "int32"), ("scatter", [8], 1001, "float64"), ("allreduce", [4, 4], 16 * 37 + 3, "int8"),
+    ("allreduce", [4, 4], 16 * 40 + 4, "int8")])
+def test_vector_items_are_dword_aligned(coll, dims, count, dtype):
+    """A tile is lowered as 16-byte vectors (ITEM_VEC) only when every leaf and
+    output offset it touches is dword-aligned and its length is a whole number
+    of vectors.  Leaves and outputs need not share a residue modulo the vector
+    (the kernels load unaligned and realign stores), so odd-count copies and
+    reductions still vectorize; int8 blocks off a dword boundary do not."""
+    size = {"int8": 1, "int32": 4, "float32": 4, "float64": 8}[dtype]
+    vec = 16 // size
+    for ex in modes(dims, coll, count, dtype):
+        meta, items, pool, launches = plan_dump(dims, coll, count, dtype, exec=ex)
+        vitems = items[(items["flags"] & 1) != 0]
+        if dtype != "int8":
+            assert len(vitems) > 0, ex
+        for it in vitems:
+            assert it["n"] % vec == 0
+            if it["flags"] & 2:  # copy run: first entry and per-copy step
+                offs = [pool[it["leaf_off"]]["off"], pool[it["leaf_off"] + 1]["off"],
+                        pool[it["out_off"]]["off"], pool[it["out_off"] + 1]["off"]]
+            else:
+                ents = np.concatenate([pool[it["leaf_off"]: it["leaf_off"] + (1 << it["lg"])],
+                                       pool[it["out_off"]: it["out_off"] + it["nouts"]]])
+                offs = [int(e) + int(it["delta"]) for e in ents["off"]]
+            assert all((int(o) * size) % 4 == 0 for o in offs), (ex, it)

@@ -47,9 +47,9 @@ def parse():
     ap.add_argument("--no-small", action="store_true", help="skip the 2x2x2 and 4x4x4 allreduce configurations")
     ap.add_argument("--no-collectives", action="store_true", help="skip the per-collective 8x8x8 runs")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
-    ap.add_argument("--only", choices=["batched", "step", "copy"], default=None,
+    ap.add_argument("--only", choices=["batched", "step", "copy", "alltoall"], default=None,
                     help="profiling aid: run only that part, --steps times, with a fixed launch shape")
-    ap.add_argument("--tuning", default="2,128,0,0", help="vpt,threads,persistent,nontemporal for --only batched")
+    ap.add_argument("--tuning", default="2,128,0,0", help="vpt,threads,persistent,nontemporal for --only batched/alltoall")
     return ap.parse_args()
 
 
@@ -98,14 +98,27 @@ def cpu_baseline(budget_s):
 
 def profile_only(args, stream, send, recv, recv_step):
     """Run one part of the benchmark for a profiler: the batched fused launch
-    with a fixed shape, the per-step launches, or a plain device copy of the
-    same 2 GiB (the reference point for HBM counters)."""
+    with a fixed shape, the per-step launches, a plain device copy of the
+    same 2 GiB (the reference point for HBM counters), or the fused alltoall
+    on the same torus (int64, 4 KiB per rank pair: copy runs only) with a
+    fixed shape."""
     import torch
     from bine_trees_fugaku_amd import Batch, VComm
     with torch.cuda.stream(stream):
         if args.only == "batched":
             comm = VComm(DIMS, exec="FUSED")
             batch = Batch(comm, [{"coll": "allreduce", "sendbuf": send, "recvbuf": recv, "count": COUNT}])
+            vpt, thr, pers, nt = (int(v) for v in args.tuning.split(","))
+            batch.set_tuning(vpt, thr, pers, nt)
+            for _ in range(args.steps):
+                batch.launch(stream)
+        elif args.only == "alltoall":
+            g = torch.Generator(device="cuda")
+            g.manual_seed(99)
+            a = torch.randint(-2**40, 2**40, (P, P * 512), dtype=torch.int64, device="cuda", generator=g)
+            b = torch.empty_like(a)
+            comm = VComm(DIMS, exec="FUSED")
+            batch = Batch(comm, [{"coll": "alltoall", "sendbuf": a, "recvbuf": b, "count": 512}])
             vpt, thr, pers, nt = (int(v) for v in args.tuning.split(","))
             batch.set_tuning(vpt, thr, pers, nt)
             for _ in range(args.steps):

@@ -191,6 +191,37 @@ def test_batch_alignment_per_bucket(gpu):
         b.close()
 
 
+def test_realigning_kernels_with_deep_trees(gpu):
+    """A float32 list that writes far more than it reads (a bcast of many outputs per leaf) runs the realigning
+    kernels; its odd-count allreduce then takes the realigned stores in both kernel paths: shallow trees, and deep
+    trees split across the waves of 128- to 512-thread workgroups.  Every candidate shape writes the oracle's
+    bytes."""
+    from bine_trees_fugaku_amd.collectives import tuning_candidates
+    P, dims = 64, [4, 4, 4]
+    cb, ca = 64 * 61 + 5, 64 * 37 + 5
+    root_row = randn((cb,), torch.float32, 8)
+    bc = torch.zeros((P, cb), dtype=torch.float32, device="cuda")
+    send = randn((P, ca), torch.float32, 9)
+    recv = torch.zeros_like(send)
+    want_ar = O.allreduce(send.cpu().numpy(), dims)
+    want_bc = np.tile(root_row.cpu().numpy().reshape(1, -1), (P, 1))
+    with VComm(dims, exec="FUSED") as comm:
+        b = Batch(comm, [{"coll": "bcast", "sendbuf": None, "recvbuf": bc, "count": cb},
+                         {"coll": "allreduce", "sendbuf": send, "recvbuf": recv, "count": ca}])
+        st = b.stats()
+        assert st["bytes_written"] >= 2 * st["bytes_read"]  # store-heavy: the realigning kernels run
+        for vpt, thr, pers, nt in tuning_candidates():
+            recv.zero_()
+            bc.zero_()
+            bc[0] = root_row
+            b.set_tuning(int(vpt), int(thr), int(pers), int(nt))
+            b.launch()
+            torch.cuda.synchronize()
+            assert recv.cpu().numpy().tobytes() == want_ar.tobytes(), (vpt, thr, pers, nt)
+            assert bc.cpu().numpy().tobytes() == want_bc.tobytes(), (vpt, thr, pers, nt)
+        b.close()
+
+
 def test_batch_rejects_bad_items(gpu):
     P = 8
     s = randn((P, 16), torch.float32, 0)

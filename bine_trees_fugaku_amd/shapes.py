@@ -114,12 +114,14 @@ class ShapeModel:
     """Predicts, per candidate shape, log2 of its launch time relative to the
     mean over shapes; picks the fastest."""
 
-    def __init__(self, n_shapes: int, hidden: int = 128, depth: int = 3, seed: int = 0, device: int = 0):
+    # 256 x 3 trained for 800 epochs: held-out regret 2.5-2.6 % over three seeds, against 4.4-4.8 % at
+    # 128 x 3 and 400 epochs, which underfit (3.5 % regret on its own training rows)
+    def __init__(self, n_shapes: int, hidden: int = 256, depth: int = 3, seed: int = 0, device: int = 0):
         self.net = sel.MLP([N_FEATURES] + [hidden] * depth + [n_shapes], max_batch=4096, seed=seed, device=device)
         self.device = device
         self.mu, self.sigma = 0.0, 1.0
 
-    def fit(self, data: Timings, epochs: int = 400, lr: float = 2e-3, seed: int = 0):
+    def fit(self, data: Timings, epochs: int = 800, lr: float = 2e-3, seed: int = 0):
         torch = self.net.torch
         # targets relative to each row's mean: the choice depends only on differences between shapes
         y = np.log2(data.T)

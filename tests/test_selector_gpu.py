@@ -154,7 +154,7 @@ def test_shape_model_regret_against_autotuner(shape_timings):
     from bine_trees_fugaku_amd import shapes as S
     train, test = shape_timings
     m = S.ShapeModel(len(train.shapes), seed=0)
-    m.fit(train, epochs=400)
+    m.fit(train)
     choice = m.choose(test.X)
     m.close()
     r = S.regret(test, choice)

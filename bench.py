@@ -385,7 +385,7 @@ def linksim_tori(stream):
 
 
 def selector_regret():
-    """Learned selector: train the 3x128 network on the GPU on 199 training tori,
+    """Learned selector: train the 3x256 network on the GPU on 199 training tori,
     measure mean regret on the five held-out tori against exhaustive simulation."""
     import numpy as np
     from bine_trees_fugaku_amd import selector as sel
@@ -394,7 +394,7 @@ def selector_regret():
     test = sel.simulate(sel.HELD_OUT, sel.default_sizes(4))
     t1 = time.perf_counter()
     s = sel.Selector(seed=0)
-    s.fit(train, epochs=60)
+    s.fit(train)
     t2 = time.perf_counter()
     r = sel.regret(test, s.choose_rows(test)[0])
     ch, unsure = s.choose_rows(test, margin=0.05)

@@ -144,6 +144,13 @@ def load():
         if _lib is None:
             if not os.path.exists(LIB_PATH):
                 raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m bine_trees_fugaku_amd.build`")
+            # PyTorch ships its own HIP runtime (its libraries need libamdhip64.so from their own directory).
+            # Loaded first, it also satisfies this library's libamdhip64.so.7, so the process has one runtime.
+            # Loaded after this library, torch would bring a second HIP/HSA runtime, which then finds no GPU.
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
             L = ctypes.CDLL(LIB_PATH)
             _sig(L)
             _lib = L

@@ -89,7 +89,7 @@ def trained(gpu):
     train = sel.simulate(sel.training_tori(), sel.default_sizes(2))
     test = sel.simulate(sel.HELD_OUT, sel.default_sizes(4))
     s = sel.Selector(seed=0)
-    losses = s.fit(train, epochs=60)
+    losses = s.fit(train)
     print(f"selector: {len(train.X)} rows, {train.samples} samples, final loss {losses[-1]:.5f}", flush=True)
     yield s, train, test, losses
     s.close()

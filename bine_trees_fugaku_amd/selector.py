@@ -7,7 +7,7 @@ ships a hand-written decision tree (fjmpi-dectree.conf).  Here a small fp32
 network (3 hidden layers of 256, ReLU), trained on the GPU by the library's
 own HIP kernels, predicts the time of every (algorithm, segments) option from
 the link-traffic simulation; exhaustive simulation is the fallback when the
-network is unsure, and the judge of its regret.
+network is unsure, and the yardstick of its regret.
 
 Cost model of one option (times in microseconds):
 

@@ -6,7 +6,7 @@ thread, workgroup size, persistent grid, non-temporal access) on the real
 buffers: about 28 x (iters + 1) launches before the first useful one.  This
 model predicts the time of every shape from a few features of the batch, so a
 new batch starts at a good shape without any timing run; the autotuner remains
-the fallback and the judge of the model's regret.
+the fallback and the yardstick of the model's regret.
 
 The network is the selector's fp32 MLP (HIP forward/backward/Adam), inputs
 below, one output per candidate shape: log2 of the launch time, standardised.

@@ -3,7 +3,7 @@
 The reference validates with INT32, count 131072, every collective on N in
 {4, 8, 64} ranks, the allreduce variants on the non-powers of two
 {6, 10, 12, 14, 18} and on the virtual tori of validate/allreduce.sh:86-139
-(SURVEY of the reference's validate/*.sh).  Here every case is compared bit
+(the cases of the reference's validate/*.sh).  Here every case is compared bit
 for bit with the oracle, in per-step and fused execution.  For N = 64 the three
 collectives whose buffers grow with N^2 (reduce_scatter, allgather, alltoall)
 use count 2048 so each case stays within a few hundred MiB.

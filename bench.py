@@ -86,7 +86,7 @@ def cpu_baseline(budget_s):
         O.allreduce(x, DIMS)
         reps += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or reps >= 50:
+        if el >= budget_s or reps >= 400:
             break
     per = el / reps
     gbps = 2.0 * P * count * ELEM / per / 1e9

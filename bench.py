@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--no-linksim", action="store_true", help="skip the link-traffic simulation run")
     ap.add_argument("--no-selector", action="store_true", help="skip training and scoring the selector")
     ap.add_argument("--no-small", action="store_true", help="skip the 2x2x2 and 4x4x4 allreduce configurations")
+    ap.add_argument("--no-share", action="store_true", help="skip the one-GPU-share (8x8, 64 ranks) allreduce")
     ap.add_argument("--no-collectives", action="store_true", help="skip the per-collective 8x8x8 runs")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--only", choices=["batched", "step", "copy", "alltoall"], default=None,
@@ -172,6 +173,37 @@ def bucketed_p64(stream, seed=7, nb=256):
            "batched_vs_per_step": round(s_wall / b_wall, 2), "per_step_launches": 13 * nb,
            "tuning": batch.tuning(), "bitwise_equal": same}
     batch.close()
+    comm.close()
+    step.close()
+    return out
+
+
+def one_gpu_share(stream, seed=64):
+    """The in-HBM part one GPU would hold if the 8x8x8 torus were sharded over 8
+    GPUs by its first coordinate: 64 ranks (an 8x8 slab) of fp64, 4 MiB per rank,
+    run here as the fused Bine allreduce over that 8x8 torus.  A proxy for the
+    per-GPU fused segments between the cross-GPU steps (same ranks, same bytes
+    per rank); checked bitwise against the per-step execution."""
+    import torch
+    from bine_trees_fugaku_amd import Batch, VComm
+    dims, count = [8, 8], COUNT
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    x = torch.randn((64, count), dtype=torch.float64, device="cuda", generator=g)
+    y, z = torch.empty_like(x), torch.empty_like(x)
+    b_alg = 2 * 64 * count * 8
+    with torch.cuda.stream(stream):
+        comm = VComm(dims, exec="FUSED")
+        bt = Batch(comm, [{"coll": "allreduce", "sendbuf": x, "recvbuf": y, "count": count}])
+        bt.autotune(iters=2, stream=stream)
+        w, _ = time_launches(lambda: bt.launch(stream), stream, 20, 3)
+        step = VComm(dims, exec="STEP")
+        step.allreduce(x, z, stream=stream)
+    torch.cuda.synchronize()
+    out = {"workload": "allreduce_8x8_fp64_4MiB_per_rank (64 ranks: one GPU's share of 8x8x8 over 8 GPUs)",
+           "algorithmic_bytes": b_alg, "batched_ms": round(w, 4), "batched_GBps": round(b_alg / (w * 1e-3) / 1e9, 1),
+           "tuning": bt.tuning(), "bitwise_equal_step": bool(torch.equal(y.view(torch.int64), z.view(torch.int64)))}
+    bt.close()
     comm.close()
     step.close()
     return out
@@ -492,6 +524,9 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_small:
         line["config"]["small_configs"] = small_configs(stream, cpu=not args.no_cpu_baseline)
+        torch.cuda.empty_cache()
+    if not args.no_share:
+        line["config"]["one_gpu_share"] = one_gpu_share(stream)
         torch.cuda.empty_cache()
     if not args.no_collectives:
         line["config"]["collectives_8x8x8"] = per_collective_8x8x8(stream)

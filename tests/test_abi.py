@@ -149,3 +149,49 @@ def test_host_only_entry_points_check_their_arguments():
     assert L.bine_mlp_forward(None, None, 1, None, None) == _lib.BINE_ERR_ARG
     assert L.bine_sim_run(None, None) == _lib.BINE_ERR_ARG
     assert L.bine_batch_time_shapes(None, None, 1, None, 1, None) == _lib.BINE_ERR_ARG
+
+
+C_CALLER = r"""
+#include <stdio.h>
+#include "bine_hip.h"
+
+int main(void) {
+    bine_vcomm_config_t cfg;
+    if (bine_vcomm_config_init(&cfg, 512) != BINE_OK) { fprintf(stderr, "%s\n", bine_last_error()); return 1; }
+    printf("dims %u %u\n", cfg.dimensions[0], cfg.dimensions_num);
+    printf("remap %u\n", bine_remap_rank(5, 8));
+    const uint32_t dims[3] = {8, 8, 8};
+    uint32_t peers[9];
+    if (bine_compute_peers(dims, 3, 0, 0, BINE_FAMILY_BINE, peers) != BINE_OK) return 2;
+    printf("peers");
+    for (int i = 0; i < 9; ++i) printf(" %u", peers[i]);
+    printf("\n");
+    return 0;
+}
+"""
+
+
+def test_c_caller_compiles_links_and_runs(tmp_path):
+    """A plain C11 program (the binding a C caller of the reference's MPI layer would write) compiles against
+    include/bine_hip.h without warnings, links libbine_hip.so and gets the reference's values from the host-only
+    entry points: remap_rank(5, 8) = 7 (simulation/remap_rank.c) and rank 0's negabinary peers on 8x8x8."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    _lib.load()  # built
+    src = tmp_path / "caller.c"
+    src.write_text(C_CALLER)
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    inc = os.path.dirname(HEADER)
+    r = subprocess.run([cc, "-std=c11", "-Wall", "-Wextra", "-Werror", "-pedantic", f"-I{inc}", str(src), "-o", str(exe),
+                        f"-L{libdir}", "-lbine_hip", f"-Wl,-rpath,{libdir}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[0] == "dims 512 1"
+    assert lines[1] == "remap 7"
+    assert lines[2] == "peers 64 8 1 448 56 7 192 24 3"

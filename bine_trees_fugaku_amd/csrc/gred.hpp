@@ -19,7 +19,8 @@ enum : uint16_t {
     ITEM_RUN = 2,  // a run of `pad` one-leaf, one-output copies (see GredItem)
 };
 
-// Longest run of copies merged into one item: two, one pass of the copy-run kernel.  On the 8x8x8 alltoall
+// Longest run of copies merged into one item: two, one pass of the copy-run kernel.  Lowering also cuts a
+// long single copy into runs of two consecutive tiles.  On the 8x8x8 alltoall
 // (int64, 4 KiB per rank pair) runs of 2 moved 6.2 TB/s, of 4 5.9, of 8 5.9, of 16 5.6 and of 32 5.3 TB/s:
 // longer runs serialise passes inside one workgroup and leave fewer, longer items; unmerged copies ran 5.75.
 constexpr uint32_t kMaxRun = 2;

@@ -1739,6 +1739,26 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
                 it.bucket = gb;
                 out->items.push_back(it);
             };
+            // two consecutive whole tiles of a long single copy as one copy run: one pass of the copy-run
+            // kernel with two copies in flight, like the runs of short copies
+            auto emit_tile_pair = [&](int64_t d0, int64_t n) {
+                tile_no.push_back(tiles++);
+                GredItem it{};
+                it.leaf_off = (uint32_t)out->pool.size();
+                out->pool.push_back(PoolEntry{g.leaves[0].rb, (uint32_t)(g.leaves[0].off + d0)});
+                out->pool.push_back(PoolEntry{0, (uint32_t)n});  // step: same rank, next tile
+                it.out_off = (uint32_t)out->pool.size();
+                out->pool.push_back(PoolEntry{g.outs[0].rb, (uint32_t)(g.outs[0].off + d0)});
+                out->pool.push_back(PoolEntry{0, (uint32_t)n});
+                it.lg = 0;
+                it.flags = ITEM_VEC | ITEM_RUN;
+                it.nouts = 1;
+                it.n = (uint32_t)n;
+                it.bucket = gb;
+                it.pad = 2;
+                out->items.push_back(it);
+            };
+            const bool long_copy = g.lg == 0 && g.leaves.size() == 1 && g.outs.size() == 1 && kMaxRun >= 2;
             int64_t d = 0;
             if (same) {
                 int64_t head = (vec - res) % vec;
@@ -1746,7 +1766,10 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
                 if (head) emit(0, head, false);
                 d = head;
                 int64_t body = ((g.n - d) / vec) * vec;
-                for (int64_t x = 0; x < body; x += tile_elems) emit(d + x, std::min(tile_elems, body - x), true);
+                int64_t x = 0;
+                if (long_copy)
+                    for (; x + 2 * tile_elems <= body; x += 2 * tile_elems) emit_tile_pair(d + x, tile_elems);
+                for (; x < body; x += tile_elems) emit(d + x, std::min(tile_elems, body - x), true);
                 d += body;
             }
             for (; d < g.n; d += tile_elems) emit(d, std::min(tile_elems, g.n - d), false);

@@ -372,7 +372,7 @@ def per_collective_8x8x8(stream):
                 fn = lambda: comm.gather(send, recv, stream=stream)  # noqa: E731
             else:
                 fn = lambda: comm.reduce(send, recv, stream=stream)  # noqa: E731
-            comm.autotune(name, send, recv, count, exec="FUSED", iters=2, stream=stream)
+            comm.autotune(name, send, recv, count, exec="FUSED", iters=5, stream=stream)
             w, ev = time_launches(fn, stream, 10, 2)
             out[name] = {"algorithmic_bytes": int(b_alg), "ms": round(ev, 4),
                          "GBps": round(b_alg / (ev * 1e-3) / 1e9, 1),

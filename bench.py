@@ -155,7 +155,7 @@ def bucketed_p64(stream, seed=7, nb=256):
         comm = VComm([4, 4, 4], exec="FUSED")
         batch = Batch(comm, [{"coll": "allreduce", "sendbuf": s, "recvbuf": r, "count": int(n)}
                              for s, r, n in zip(sends, recvs, sizes)])
-        batch.autotune(iters=2, stream=stream)
+        batch.autotune(iters=3, stream=stream)
         b_wall, _ = time_launches(lambda: batch.launch(stream), stream, 5, 2)
         g_wall, _ = time_launches(lambda: batch.launch_graph(stream), stream, 5, 2)
         step = VComm([4, 4, 4], exec="STEP")
@@ -195,7 +195,7 @@ def one_gpu_share(stream, seed=64):
     with torch.cuda.stream(stream):
         comm = VComm(dims, exec="FUSED")
         bt = Batch(comm, [{"coll": "allreduce", "sendbuf": x, "recvbuf": y, "count": count}])
-        bt.autotune(iters=2, stream=stream)
+        bt.autotune(iters=5, stream=stream)
         w, _ = time_launches(lambda: bt.launch(stream), stream, 20, 3)
         step = VComm(dims, exec="STEP")
         step.allreduce(x, z, stream=stream)
@@ -245,7 +245,7 @@ def mixed_8x8x8(stream, seed=21):
     with torch.cuda.stream(stream):
         comm = VComm(DIMS, exec="FUSED")
         batch = Batch(comm, [{"coll": c, "sendbuf": s, "recvbuf": r, "count": n} for c, (s, r, n) in t.items()])
-        batch.autotune(iters=2, stream=stream)
+        batch.autotune(iters=5, stream=stream)
         b_wall, _ = time_launches(lambda: batch.launch(stream), stream, 10, 3)
         g_wall, _ = time_launches(lambda: batch.launch_graph(stream), stream, 10, 3)
         outs = {c: r.clone() for c, (s, r, n) in t.items() if c != "bcast"}

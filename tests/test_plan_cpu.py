@@ -259,3 +259,27 @@ def test_vector_items_are_dword_aligned(coll, dims, count, dtype):
                                        pool[it["out_off"]: it["out_off"] + it["nouts"]]])
                 offs = [int(e) + int(it["delta"]) for e in ents["off"]]
             assert all((int(o) * size) % 4 == 0 for o in offs), (ex, it)
+
+
+@pytest.mark.parametrize("coll,count", [("scatter", 64 * 5 + 12), ("gather", 64 * 6 + 4), ("scatter", 64 * 2),
+                                        ("gather", 64 * 3 + 5)])
+def test_long_copies_lower_to_tile_pairs(coll, count):
+    """A single copy of at least two tiles is lowered as copy runs of two consecutive tiles (first entry plus a
+    step of one tile, same rank); odd tile counts and ragged tails stay ordinary items.  The emulated result
+    equals the oracle."""
+    P, dims, tile = 8, [8], 64
+    x = data((P, count), "int32", count)
+    meta, items, pool, launches = plan_dump(dims, coll, count, "int32", exec="FUSED", tile_elems=tile)
+    runs = items[(items["flags"] & 2) != 0]
+    pairs = [it for it in runs if pool[it["leaf_off"] + 1]["rb"] == 0 and pool[it["out_off"] + 1]["rb"] == 0]
+    assert len(pairs) > 0
+    for it in pairs:  # two consecutive whole tiles of one block
+        assert it["n"] == tile and it["pad"] == 2 and it["flags"] & 1
+        assert pool[it["leaf_off"] + 1]["off"] == tile and pool[it["out_off"] + 1]["off"] == tile
+    if coll == "scatter":
+        send = x.reshape(-1)
+        _, bufs = run_plan(dims, coll, count, "int32", "FUSED", send=send, tile=tile, root=0)
+        assert same_bits(bufs[E.BUF_RECV], O.scatter(send, P, 0))
+    else:
+        _, bufs = run_plan(dims, coll, count, "int32", "FUSED", send=x, tile=tile, root=0)
+        assert same_bits(bufs[E.BUF_RECV][0], O.gather(x, 0))

@@ -147,6 +147,18 @@ def test_mixed_collectives_8x8x8(gpu, scale):
         b.launch_graph()
         torch.cuda.synchronize()
         got = {c: r.clone() for c, (s, r, n) in t.items()}
+        # replay the captured graph and launch directly again, with eager work in between: every result must
+        # match the first
+        for k, how in enumerate(("graph", "graph", "launch", "graph")):
+            t["bcast"][1].copy_(bcast_in)
+            for c, (s, r, n) in t.items():
+                if c != "bcast":
+                    r.fill_(float("nan") if r.dtype.is_floating_point else -1)
+            torch.cuda.synchronize()
+            b.launch_graph() if how == "graph" else b.launch()
+            torch.cuda.synchronize()
+            for c, (s, r, n) in t.items():
+                assert same_bits(got[c], r), (k, how, c)
         b.close()
         # single calls on fresh outputs
         t["bcast"][1].copy_(bcast_in)

@@ -48,9 +48,9 @@ def parse():
     ap.add_argument("--no-share", action="store_true", help="skip the one-GPU-share (8x8, 64 ranks) allreduce")
     ap.add_argument("--no-collectives", action="store_true", help="skip the per-collective 8x8x8 runs")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
-    ap.add_argument("--only", choices=["batched", "step", "copy", "alltoall"], default=None,
+    ap.add_argument("--only", choices=["batched", "step", "copy", "alltoall", "buckets"], default=None,
                     help="profiling aid: run only that part, --steps times, with a fixed launch shape")
-    ap.add_argument("--tuning", default="2,128,0,0", help="vpt,threads,persistent,nontemporal for --only batched/alltoall")
+    ap.add_argument("--tuning", default="2,128,0,0", help="vpt,threads,persistent,nontemporal for --only batched/alltoall/buckets")
     return ap.parse_args()
 
 
@@ -100,9 +100,9 @@ def cpu_baseline(budget_s):
 def profile_only(args, stream, send, recv, recv_step):
     """Run one part of the benchmark for a profiler: the batched fused launch
     with a fixed shape, the per-step launches, a plain device copy of the
-    same 2 GiB (the reference point for HBM counters), or the fused alltoall
-    on the same torus (int64, 4 KiB per rank pair: copy runs only) with a
-    fixed shape."""
+    same 2 GiB (the reference point for HBM counters), the fused alltoall
+    on the same torus (int64, 4 KiB per rank pair: copy runs only), or the
+    256 uneven fp32 buckets at P = 64 of the bucketed leg, with a fixed shape."""
     import torch
     from bine_trees_fugaku_amd import Batch, VComm
     with torch.cuda.stream(stream):
@@ -120,6 +120,21 @@ def profile_only(args, stream, send, recv, recv_step):
             b = torch.empty_like(a)
             comm = VComm(DIMS, exec="FUSED")
             batch = Batch(comm, [{"coll": "alltoall", "sendbuf": a, "recvbuf": b, "count": 512}])
+            vpt, thr, pers, nt = (int(v) for v in args.tuning.split(","))
+            batch.set_tuning(vpt, thr, pers, nt)
+            for _ in range(args.steps):
+                batch.launch(stream)
+        elif args.only == "buckets":
+            import numpy as np
+            rng = np.random.default_rng(7)
+            sizes = np.exp(rng.uniform(np.log(4096), np.log(8 << 20), 256)).astype(np.int64) // 4
+            g = torch.Generator(device="cuda")
+            g.manual_seed(7)
+            sends = [torch.randn((64, int(n)), dtype=torch.float32, device="cuda", generator=g) for n in sizes]
+            recvs = [torch.empty_like(x) for x in sends]
+            comm = VComm([4, 4, 4], exec="FUSED")
+            batch = Batch(comm, [{"coll": "allreduce", "sendbuf": x, "recvbuf": y, "count": int(n)}
+                                 for x, y, n in zip(sends, recvs, sizes)])
             vpt, thr, pers, nt = (int(v) for v in args.tuning.split(","))
             batch.set_tuning(vpt, thr, pers, nt)
             for _ in range(args.steps):

@@ -15,13 +15,19 @@
  * Pinning.  Every function is checked by the tests against closed forms,
  * invariants, brute force, MPI semantics, the reference's own
  * simulation/remap_rank.c (built under _ref/ and as a committed fixture) or
- * the oracle's own message traces.  Parity unpinned: the fp association and
- * block ownership that or_allreduce / or_get_tree / or_block_step produce on
- * NON-power-of-two sizes (where compute_block_step's tie-break and the
- * "disconnect myself" rule act, lib/libbine_common.cc:1499-1500, 1574, 2045)
- * follow the code as read but are confirmed by no reference output; and
- * MPI_Reduce_local for fp32/fp64 is taken as IEEE a + b (the system MPI is
- * not part of the reference).
+ * the oracle's own message traces.  On power-of-two sizes the floating-point
+ * association of every reducing collective (or_allreduce B / B_COALESCE /
+ * B_CONT / L with 1-6 ports, or_reduce_scatter in both distance orders,
+ * or_reduce as binomial tree and as reduce-scatter + gather) is checked bit
+ * for bit against replays of the reduction trees built from the peer
+ * function and remap_rank alone (tests/test_oracle_association.py).
+ * Parity unpinned: the fp association and block ownership that or_allreduce
+ * / or_get_tree / or_block_step produce on NON-power-of-two sizes (where
+ * compute_block_step's tie-break and the "disconnect myself" rule act,
+ * lib/libbine_common.cc:1499-1500, 1574, 2045, and the shrink/enlarge folds of
+ * allreduce L, :269-353) follow the code as read but are confirmed by no
+ * reference output; and MPI_Reduce_local for fp32/fp64 is taken as IEEE
+ * a + b (the system MPI is not part of the reference).
  */
 #ifndef BINE_ORACLE_H
 #define BINE_ORACLE_H

@@ -10,7 +10,8 @@ MPI at run time, bench/bench.cc:270,328-344), so the oracle is pinned by:
   * invariants that fix the schedule uniquely on power-of-two tori
     (|S| = P/2^(s+1), S and R disjoint, S_r = R_peer, each step a matching);
   * an independent replay of the reduction tree built from the peer function
-    alone, which fixes the floating-point association of allreduce B.
+    alone, which fixes the floating-point association of allreduce B (the
+    other reducing collectives: test_oracle_association.py).
 """
 import math
 import os

@@ -18,10 +18,11 @@ STATUS_NAMES = {0: "BINE_OK", 1: "BINE_ERR_ARG", 2: "BINE_ERR_UNSUPPORTED", 3: "
 
 EXPORTED = [
     "bine_vcomm_config_init", "bine_vcomm_config_from_env", "bine_vcomm_create", "bine_vcomm_destroy",
-    "bine_vcomm_size", "bine_vcomm_set_exec", "bine_allreduce", "bine_reduce_scatter", "bine_allgather",
+    "bine_vcomm_size", "bine_vcomm_set_exec", "bine_vcomm_set_cache_limit", "bine_vcomm_trim",
+    "bine_vcomm_cache_info", "bine_allreduce", "bine_reduce_scatter", "bine_allgather",
     "bine_bcast", "bine_alltoall", "bine_scatter", "bine_gather", "bine_reduce", "bine_compute_peers",
     "bine_compute_step_info", "bine_block_step", "bine_get_tree", "bine_remap_rank", "bine_reduce_local",
-    "bine_plan_stats", "bine_plan_autotune", "bine_plan_dump", "bine_batch_create", "bine_batch_launch", "bine_batch_launch_graph",
+    "bine_reduce_local3",    "bine_plan_stats", "bine_plan_autotune", "bine_plan_dump", "bine_batch_create", "bine_batch_launch", "bine_batch_launch_graph",
     "bine_batch_autotune", "bine_batch_stats", "bine_batch_get_tuning", "bine_batch_set_tuning",
     "bine_batch_destroy", "bine_tuning_candidates", "bine_batch_time_shapes", "bine_sim_classes", "bine_sim_peers", "bine_sim_create", "bine_sim_run", "bine_sim_case_info",
     "bine_sim_class_stats", "bine_sim_link_counts", "bine_sim_destroy", "bine_mlp_create", "bine_mlp_num_params",
@@ -84,6 +85,9 @@ def _sig(L):
     L.bine_vcomm_destroy.argtypes = [vp]
     L.bine_vcomm_size.argtypes = [vp, pu32]
     L.bine_vcomm_set_exec.argtypes = [vp, i32]
+    L.bine_vcomm_set_cache_limit.argtypes = [vp, u32]
+    L.bine_vcomm_trim.argtypes = [vp]
+    L.bine_vcomm_cache_info.argtypes = [vp, pu32, ctypes.POINTER(ctypes.c_uint64)]
     L.bine_allreduce.argtypes = [vp, vp, i32, i32, i32, vp, vp]
     L.bine_reduce_scatter.argtypes = [vp, vp, pint, i32, i32, vp, vp]
     L.bine_allgather.argtypes = [vp, i32, i32, vp, i32, i32, vp, vp]
@@ -99,9 +103,10 @@ def _sig(L):
     L.bine_remap_rank.argtypes = [u32, u32]
     L.bine_remap_rank.restype = u32
     L.bine_reduce_local.argtypes = [vp, vp, i32, i32, i32, vp]
+    L.bine_reduce_local3.argtypes = [vp, vp, vp, i32, i32, i32, vp]
     L.bine_plan_stats.argtypes = [vp, i32, i32, i32, i32, i32, i32, ctypes.POINTER(PlanStats)]
     L.bine_plan_autotune.argtypes = [vp, i32, vp, vp, i32, i32, i32, i32, i32, vp, i32]
-    L.bine_plan_dump.argtypes = [ctypes.POINTER(VCommConfig), i32, i32, i32, i32, i32, i32, ctypes.c_int64,
+    L.bine_plan_dump.argtypes = [ctypes.POINTER(VCommConfig), i32, i32, i32, i32, i32, i32, i32, ctypes.c_int64,
                                  ctypes.POINTER(PlanDumpInfo), vp, vp, pu32]
     L.bine_batch_create.argtypes = [vp, ctypes.POINTER(BatchItem), i32, ctypes.POINTER(vp)]
     L.bine_batch_launch.argtypes = [vp, vp]

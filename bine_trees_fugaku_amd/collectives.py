@@ -163,7 +163,7 @@ class VComm:
 
     # ---- MPI entry points (lib/libbine.cc) --------------------------------------
     def allreduce(self, sendbuf, recvbuf, op: str = "SUM", stream=None):
-        """MPI_Allreduce (lib/libbine.cc:1012)."""
+        """MPI_Allreduce (lib/libbine.cc:1012).  sendbuf is recvbuf: MPI_IN_PLACE (per-step schedule)."""
         count = self._per_rank(sendbuf)
         self._expect(recvbuf, self.size * count, "recvbuf", sendbuf)
         _lib.check(self._lib.bine_allreduce(sendbuf.data_ptr(), recvbuf.data_ptr(), count, _torch_dtype_code(sendbuf),
@@ -224,6 +224,35 @@ class VComm:
         _lib.check(self._lib.bine_reduce(sendbuf.data_ptr(), recvbuf.data_ptr(), c, _torch_dtype_code(sendbuf),
                                          OP[op], root, self.handle, self._stream(stream)), "bine_reduce")
 
+    # ---- local reductions (lib/libbine_coll.cc:153-267) ------------------------
+    def reduce_local(self, inbuf, inoutbuf, op: str = "SUM", stream=None):
+        """reduce_local, 2-argument form: inoutbuf = inbuf + inoutbuf (one partition, any length)."""
+        self._expect(inoutbuf, inbuf.numel(), "inoutbuf", inbuf)
+        _lib.check(self._lib.bine_reduce_local(inbuf.data_ptr(), inoutbuf.data_ptr(), inbuf.numel(),
+                                               _torch_dtype_code(inbuf), OP[op], self._stream(stream)),
+                   "bine_reduce_local")
+
+    def reduce_local3(self, a, b, out, op: str = "SUM", stream=None):
+        """reduce_local, 3-argument form: out = a + b."""
+        self._expect(b, a.numel(), "inbuf_b", a)
+        self._expect(out, a.numel(), "outbuf", a)
+        _lib.check(self._lib.bine_reduce_local3(a.data_ptr(), b.data_ptr(), out.data_ptr(), a.numel(),
+                                                _torch_dtype_code(a), OP[op], self._stream(stream)),
+                   "bine_reduce_local3")
+
+    # ---- plan cache --------------------------------------------------------------
+    def set_cache_limit(self, max_plans: int):
+        _lib.check(self._lib.bine_vcomm_set_cache_limit(self.handle, max_plans), "bine_vcomm_set_cache_limit")
+
+    def trim(self):
+        _lib.check(self._lib.bine_vcomm_trim(self.handle), "bine_vcomm_trim")
+
+    def cache_info(self) -> dict:
+        n, t = ctypes.c_uint32(), ctypes.c_uint64()
+        _lib.check(self._lib.bine_vcomm_cache_info(self.handle, ctypes.byref(n), ctypes.byref(t)),
+                   "bine_vcomm_cache_info")
+        return {"plans": n.value, "tmp_bytes": t.value}
+
     # ---- schedule introspection ----------------------------------------------
     def plan_stats(self, coll: str, count: int, dtype, op: str = "SUM", root: int = 0, exec: str = "AUTO") -> dict:
         st = _lib.PlanStats()
@@ -255,16 +284,16 @@ def _dtype_code(dtype) -> int:
 
 
 def plan_dump(dims: Sequence[int], coll: str, count: int, dtype, op: str = "SUM", root: int = 0, exec: str = "AUTO",
-              num_ports: int = 1, tile_elems: int = 0, **overrides):
+              num_ports: int = 1, tile_elems: int = 0, in_place: bool = False, **overrides):
     """Compile a schedule on the host (no GPU) and return its lowered work as
     numpy arrays: (info dict, items structured array, pool structured array,
-    launches (n, 2) array)."""
+    launches (n, 2) array).  in_place: the allreduce schedule for sendbuf == recvbuf."""
     import numpy as np
     L = _lib.load()
     cfg = make_config(dims, num_ports=num_ports, **overrides)
     info = _lib.PlanDumpInfo()
-    args = (ctypes.byref(cfg), COLL[coll], count, _dtype_code(dtype), OP[op], root, EXEC[exec], tile_elems,
-            ctypes.byref(info))
+    args = (ctypes.byref(cfg), COLL[coll], count, _dtype_code(dtype), OP[op], root, 1 if in_place else 0, EXEC[exec],
+            tile_elems, ctypes.byref(info))
     _lib.check(L.bine_plan_dump(*args, None, None, None), "bine_plan_dump")
     item_dt = np.dtype([("leaf_off", "<u4"), ("out_off", "<u4"), ("lg", "<u2"), ("flags", "<u2"), ("nouts", "<u4"),
                         ("delta", "<u4"), ("n", "<u4"), ("bucket", "<u4"), ("pad", "<u4")])

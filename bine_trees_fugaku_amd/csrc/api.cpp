@@ -27,7 +27,7 @@ hipError_t launch_gred(int dtype, int max_lg, const GredItem* items, uint32_t n,
                        const LaunchShape& sh, hipStream_t st);
 hipError_t launch_grun(uint32_t esize, const GredItem* items, uint32_t n, const PoolEntry* pool, const BufTable& tab,
                        const BufTable* tabs, bool batch, uint32_t allow_vec, const LaunchShape& sh, hipStream_t st);
-hipError_t launch_reduce_local(int dtype, const void* in, void* inout, uint64_t n, hipStream_t st);
+hipError_t launch_reduce_local(int dtype, const void* a, const void* b, void* out, uint64_t n, hipStream_t st);
 }  // namespace bine
 
 using namespace bine;
@@ -74,6 +74,53 @@ struct DevArrays {
     }
 };
 
+// Device arrays that a HIP graph captured earlier may still name: kept until their owner goes away.
+struct Retired {
+    std::vector<void*> ptrs;
+    void keep(DevArrays* d) {
+        if (d->items) ptrs.push_back(d->items);
+        if (d->pool) ptrs.push_back(d->pool);
+        d->items = nullptr;
+        d->pool = nullptr;
+    }
+    ~Retired() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+// Copy a lowered work list to the device.  The new arrays replace d's only once every allocation and copy
+// has succeeded; on failure d is left as it was.  The old arrays are freed, or kept in `retire` when a
+// captured graph may name them.
+int install(Lowered&& low, int max_lg, int64_t tile_elems, const LaunchShape& sh, DevArrays* d, Retired* retire) {
+    DevArrays n;
+    n.low = std::move(low);
+    n.max_lg = max_lg;
+    n.tile_elems = tile_elems;
+    n.shape = sh;
+    hipError_t e = hipSuccess;
+    if (!n.low.items.empty()) {
+        e = hipMalloc(&n.items, n.low.items.size() * sizeof(GredItem));
+        if (e == hipSuccess)
+            e = hipMemcpy(n.items, n.low.items.data(), n.low.items.size() * sizeof(GredItem), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && !n.low.pool.empty()) {
+        e = hipMalloc(&n.pool, n.low.pool.size() * sizeof(PoolEntry));
+        if (e == hipSuccess)
+            e = hipMemcpy(n.pool, n.low.pool.data(), n.low.pool.size() * sizeof(PoolEntry), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        n.release();
+        return fail(BINE_ERR_HIP, std::string("work-list upload: ") + hipGetErrorString(e));
+    }
+    if (retire) retire->keep(d);
+    d->release();
+    *d = std::move(n);
+    n.items = nullptr;
+    n.pool = nullptr;
+    return BINE_OK;
+}
+
 int max_lg_of(const std::vector<std::vector<Group>>& gs) {
     int m = 0;
     for (const auto& v : gs)
@@ -81,22 +128,11 @@ int max_lg_of(const std::vector<std::vector<Group>>& gs) {
     return m;
 }
 
-int upload(const std::vector<std::vector<Group>>& gs, size_t ds, const LaunchShape& sh, uint32_t bucket, DevArrays* d) {
-    d->release();
-    lower_groups(gs, shallow_tile_elems(sh, ds), (int)(16 / ds), bucket, ds, &d->low, deep_tile_elems(sh, ds),
-                 split_lg(sh));
-    d->max_lg = max_lg_of(gs);
-    d->tile_elems = shallow_tile_elems(sh, ds);
-    d->shape = sh;
-    if (!d->low.items.empty()) {
-        HIPCHK(hipMalloc(&d->items, d->low.items.size() * sizeof(GredItem)));
-        HIPCHK(hipMemcpy(d->items, d->low.items.data(), d->low.items.size() * sizeof(GredItem), hipMemcpyHostToDevice));
-    }
-    if (!d->low.pool.empty()) {
-        HIPCHK(hipMalloc(&d->pool, d->low.pool.size() * sizeof(PoolEntry)));
-        HIPCHK(hipMemcpy(d->pool, d->low.pool.data(), d->low.pool.size() * sizeof(PoolEntry), hipMemcpyHostToDevice));
-    }
-    return BINE_OK;
+int upload(const std::vector<std::vector<Group>>& gs, size_t ds, const LaunchShape& sh, uint32_t bucket, DevArrays* d,
+           Retired* retire = nullptr) {
+    Lowered low;
+    lower_groups(gs, shallow_tile_elems(sh, ds), (int)(16 / ds), bucket, ds, &low, deep_tile_elems(sh, ds), split_lg(sh));
+    return install(std::move(low), max_lg_of(gs), shallow_tile_elems(sh, ds), sh, d, retire);
 }
 
 struct DevPlan {
@@ -104,27 +140,39 @@ struct DevPlan {
     int dtype = 0;
     size_t ds = 0;
     DevArrays step, fused;
+    Retired retired;                      // work lists replaced by an autotune (graphs may name them)
     void* tmp[2] = {nullptr, nullptr};  // BUF_TMP, BUF_TMP2 workspaces, allocated on the first per-step run
     size_t tmp_bytes[2] = {0, 0};         // 0: the per-step schedule never touches that buffer
+    // Per-step runs that use the temporaries are ordered across streams: each run waits for the event the
+    // previous one recorded (outside graph capture).
+    hipEvent_t tmp_done = nullptr;
+    bool tmp_recorded = false;
+    uint64_t last_use = 0;  // LRU stamp
+    int pins = 0;           // batches that hold this plan (never evicted while > 0)
     ~DevPlan() {
         step.release();
         fused.release();
         for (void* p : tmp)
             if (p) (void)hipFree(p);
+        if (tmp_done) (void)hipEventDestroy(tmp_done);
     }
+    size_t tmp_total() const { return tmp_bytes[0] + tmp_bytes[1]; }
 };
 
 struct PlanKey {
     int coll, dtype, op, root;
     int64_t count;
+    bool in_place;
     bool operator==(const PlanKey& o) const {
-        return coll == o.coll && dtype == o.dtype && op == o.op && root == o.root && count == o.count;
+        return coll == o.coll && dtype == o.dtype && op == o.op && root == o.root && count == o.count &&
+               in_place == o.in_place;
     }
 };
 struct PlanKeyHash {
     size_t operator()(const PlanKey& k) const {
         uint64_t h = (uint64_t)k.count * 0x9e3779b97f4a7c15ull;
-        h ^= ((uint64_t)k.coll << 48) ^ ((uint64_t)k.dtype << 40) ^ ((uint64_t)k.op << 32) ^ (uint32_t)k.root;
+        h ^= ((uint64_t)k.coll << 48) ^ ((uint64_t)k.dtype << 40) ^ ((uint64_t)k.op << 32) ^ (uint32_t)k.root ^
+             ((uint64_t)k.in_place << 56);
         return (size_t)(h ^ (h >> 29));
     }
 };
@@ -141,24 +189,42 @@ int bine::record_error(int code, const std::string& msg) {
     return code;
 }
 
+// Default bound of a vcomm's plan cache (bine_vcomm_set_cache_limit changes it).
+constexpr uint32_t kDefaultMaxPlans = 4096;
+
 struct bine_vcomm_s {
     bine_vcomm_config_t cfg;
     Torus t;
     std::mutex mu;
-    std::unordered_map<PlanKey, std::unique_ptr<DevPlan>, PlanKeyHash> plans;
+    // Compiled schedules, least recently used evicted beyond max_plans.  A batch or a call in flight holds
+    // its plan through the shared pointer, so eviction never frees a plan still in use by the host.
+    std::unordered_map<PlanKey, std::shared_ptr<DevPlan>, PlanKeyHash> plans;
+    uint32_t max_plans = kDefaultMaxPlans;
+    uint64_t clock = 0;
 };
 
 namespace {
 
+// Drop least recently used plans until at most `keep` remain (caller holds comm->mu).
+void evict_lru(bine_vcomm_t comm, size_t keep) {
+    while (comm->plans.size() > keep) {
+        auto victim = comm->plans.begin();
+        for (auto it = comm->plans.begin(); it != comm->plans.end(); ++it)
+            if (it->second->last_use < victim->second->last_use) victim = it;
+        comm->plans.erase(victim);
+    }
+}
+
 // Compile (or fetch) the schedule for a request.
-int get_plan(bine_vcomm_t comm, const CollRequest& req, int dtype, DevPlan** out) {
+int get_plan(bine_vcomm_t comm, const CollRequest& req, int dtype, std::shared_ptr<DevPlan>* out) {
     const size_t ds = dtsize(dtype);
     if (!ds) return fail(BINE_ERR_ARG, "unknown datatype");
-    PlanKey key{req.coll, dtype, req.op, req.root, req.count};
+    PlanKey key{req.coll, dtype, req.op, req.root, req.count, req.in_place};
     std::lock_guard<std::mutex> lock(comm->mu);
     auto it = comm->plans.find(key);
     if (it != comm->plans.end()) {
-        *out = it->second.get();
+        it->second->last_use = ++comm->clock;
+        *out = it->second;
         return BINE_OK;
     }
     HIPCHK(hipSetDevice(comm->cfg.device));
@@ -167,8 +233,11 @@ int get_plan(bine_vcomm_t comm, const CollRequest& req, int dtype, DevPlan** out
     if (!build_program(comm->t, comm->cfg, req, &prog, &err)) return fail(BINE_ERR_UNSUPPORTED, err);
     for (int b = 0; b < kNumBufs; ++b)
         if ((uint64_t)prog.extent[b] >= (1ull << 32)) return fail(BINE_ERR_UNSUPPORTED, "per-rank buffer exceeds 2^32 elements");
-    auto dp = std::make_unique<DevPlan>();
-    if (!compile_program(prog, true, &dp->cp, &err)) return fail(BINE_ERR_UNSUPPORTED, err);
+    auto dp = std::make_shared<DevPlan>();
+    if (!compile_program(prog, true, &dp->cp, &err)) {
+        if (req.in_place) err = "schedule cannot run in place (sendbuf == recvbuf): " + err;
+        return fail(BINE_ERR_UNSUPPORTED, err);
+    }
     dp->dtype = dtype;
     dp->ds = ds;
     int rc = upload(dp->cp.step_groups, ds, default_shape(), 0, &dp->step);
@@ -192,7 +261,9 @@ int get_plan(bine_vcomm_t comm, const CollRequest& req, int dtype, DevPlan** out
         }
     for (int b = BUF_TMP; b <= BUF_TMP2; ++b)
         if (prog.extent[b] > 0 && used[b]) dp->tmp_bytes[b - BUF_TMP] = (size_t)prog.extent[b] * ds * (prog.single[b] ? 1 : prog.P);
-    *out = dp.get();
+    dp->last_use = ++comm->clock;
+    *out = dp;
+    if (comm->max_plans > 0) evict_lru(comm, comm->max_plans - 1);
     comm->plans.emplace(key, std::move(dp));
     return BINE_OK;
 }
@@ -206,9 +277,32 @@ uint64_t kernel_launches(const Lowered& low) {
     return n;
 }
 
+// Release the temporaries of cached plans that nothing but the cache holds, least recently used first,
+// until `need` bytes were freed (caller holds comm->mu).  Returns the bytes freed.
+size_t release_idle_tmp(bine_vcomm_t comm, size_t need, const DevPlan* except) {
+    std::vector<DevPlan*> idle;
+    for (auto& kv : comm->plans)
+        if (kv.second.get() != except && kv.second.use_count() == 1 && (kv.second->tmp[0] || kv.second->tmp[1]))
+            idle.push_back(kv.second.get());
+    std::sort(idle.begin(), idle.end(), [](const DevPlan* a, const DevPlan* b) { return a->last_use < b->last_use; });
+    size_t freed = 0;
+    for (DevPlan* p : idle) {
+        if (freed >= need) break;
+        for (int i = 0; i < 2; ++i)
+            if (p->tmp[i]) {
+                (void)hipFree(p->tmp[i]);
+                p->tmp[i] = nullptr;
+                freed += p->tmp_bytes[i];
+            }
+        p->tmp_recorded = false;
+    }
+    return freed;
+}
+
 // Allocate the per-step schedule's temporaries (the fused schedule needs none).
 // Every rank keeps the reference's full-size temporary, so per-step execution
 // of large rooted collectives can need P times the reference's per-process memory.
+// When the device is out of memory, idle plans' temporaries are released and the allocation retried.
 int ensure_tmp(bine_vcomm_t comm, DevPlan* dp) {
     std::lock_guard<std::mutex> lock(comm->mu);
     for (int i = 0; i < 2; ++i) {
@@ -216,13 +310,28 @@ int ensure_tmp(bine_vcomm_t comm, DevPlan* dp) {
         HIPCHK(hipSetDevice(comm->cfg.device));
         hipError_t e = hipMalloc(&dp->tmp[i], dp->tmp_bytes[i]);
         if (e != hipSuccess) {
+            (void)hipGetLastError();
+            dp->tmp[i] = nullptr;
+            if (release_idle_tmp(comm, dp->tmp_bytes[i], dp) > 0) e = hipMalloc(&dp->tmp[i], dp->tmp_bytes[i]);
+        }
+        if (e != hipSuccess) {
             dp->tmp[i] = nullptr;
             (void)hipGetLastError();
             return fail(BINE_ERR_HIP, "per-step execution needs " + std::to_string(dp->tmp_bytes[i] >> 20) +
                                           " MiB of temporaries: " + hipGetErrorString(e));
         }
     }
+    if (dp->tmp_total() && !dp->tmp_done) HIPCHK(hipEventCreateWithFlags(&dp->tmp_done, hipEventDisableTiming));
     return BINE_OK;
+}
+
+bool capturing(hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return cs != hipStreamCaptureStatusNone;
 }
 
 // The device allocation holding [p, p + bytes) must contain the whole range.
@@ -304,31 +413,51 @@ int launch_arrays(const DevArrays& d, int dtype, const BufTable& tab, uint32_t a
     return BINE_OK;
 }
 
-// Run a collective call end to end.
-int run_coll(bine_vcomm_t comm, const CollRequest& req, int dtype, const void* send, void* recv, void* stream) {
-    if (!comm) return fail(BINE_ERR_ARG, "NULL vcomm");
-    DevPlan* dp = nullptr;
-    int rc = get_plan(comm, req, dtype, &dp);
+// Launch a per-step work list whose temporaries the plan owns.  Outside graph capture the launches wait for
+// the previous per-step run of the same plan (on whatever stream it ran) and record their own end, so calls on
+// different streams never share the temporaries at once.  Under capture the caller orders the graph.
+int launch_step_ordered(bine_vcomm_t comm, DevPlan* dp, const BufTable& tab, hipStream_t st) {
+    const uint32_t av = vec_ok(tab);
+    if (!dp->tmp_total() || capturing(st)) return launch_arrays(dp->step, dp->dtype, tab, av, st);
+    std::lock_guard<std::mutex> lock(comm->mu);
+    if (dp->tmp_recorded) HIPCHK(hipStreamWaitEvent(st, dp->tmp_done, 0));
+    int rc = launch_arrays(dp->step, dp->dtype, tab, av, st);
     if (rc) return rc;
-    const size_t sbytes = partition_bytes(dp, BUF_SEND), rbytes = partition_bytes(dp, BUF_RECV);
+    HIPCHK(hipEventRecord(dp->tmp_done, st));
+    dp->tmp_recorded = true;
+    return BINE_OK;
+}
+
+// Run a collective call end to end.  sendbuf == recvbuf (allreduce only) is MPI_IN_PLACE: the input is read
+// from recvbuf by a schedule compiled and race-checked for that aliasing.
+int run_coll(bine_vcomm_t comm, const CollRequest& req0, int dtype, const void* send, void* recv, void* stream) {
+    if (!comm) return fail(BINE_ERR_ARG, "NULL vcomm");
+    CollRequest req = req0;
+    std::shared_ptr<DevPlan> dpp;
+    int rc = get_plan(comm, req, dtype, &dpp);
+    if (rc) return rc;
+    size_t sbytes = partition_bytes(dpp.get(), BUF_SEND), rbytes = partition_bytes(dpp.get(), BUF_RECV);
+    if (overlap(send, sbytes, recv, rbytes)) {
+        if (!(req.coll == BINE_COLL_ALLREDUCE && send == recv && sbytes == rbytes))
+            return fail(BINE_ERR_ARG, "sendbuf and recvbuf overlap");
+        req.in_place = true;
+        if ((rc = get_plan(comm, req, dtype, &dpp))) return rc;
+        sbytes = 0;
+    }
+    DevPlan* dp = dpp.get();
     if ((rc = check_range(send, sbytes, "sendbuf"))) return rc;
     if ((rc = check_range(recv, rbytes, "recvbuf"))) return rc;
     int exec = comm->cfg.exec;
-    const bool alias = overlap(send, sbytes, recv, rbytes);
-    if (alias) {
-        if (!(req.coll == BINE_COLL_ALLREDUCE && send == recv && sbytes == rbytes))
-            return fail(BINE_ERR_ARG, "sendbuf and recvbuf overlap");
-        if (exec == BINE_EXEC_FUSED) return fail(BINE_ERR_UNSUPPORTED, "fused execution cannot run in place");
-        exec = BINE_EXEC_STEP;
-    }
+    if (req.in_place && exec == BINE_EXEC_FUSED)
+        return fail(BINE_ERR_UNSUPPORTED, "fused execution cannot run in place (sendbuf == recvbuf)");
     if (exec == BINE_EXEC_FUSED && !dp->cp.fused_ok)
         return fail(BINE_ERR_UNSUPPORTED, "schedule cannot be fused: " + dp->cp.fused_why);
     if (exec == BINE_EXEC_AUTO) exec = dp->cp.fused_ok ? BINE_EXEC_FUSED : BINE_EXEC_STEP;
     if (exec == BINE_EXEC_STEP && (rc = ensure_tmp(comm, dp))) return rc;
     HIPCHK(hipSetDevice(comm->cfg.device));
-    BufTable tab = make_table(dp, send, recv);
-    const uint32_t av = vec_ok(tab);
-    return launch_arrays(exec == BINE_EXEC_FUSED ? dp->fused : dp->step, dtype, tab, av, (hipStream_t)stream);
+    BufTable tab = make_table(dp, req.in_place ? recv : send, recv);
+    if (exec == BINE_EXEC_STEP) return launch_step_ordered(comm, dp, tab, (hipStream_t)stream);
+    return launch_arrays(dp->fused, dtype, tab, vec_ok(tab), (hipStream_t)stream);
 }
 
 int parse_family(const char* s, int* out) {
@@ -474,6 +603,35 @@ extern "C" int bine_vcomm_set_exec(bine_vcomm_t comm, int32_t exec) {
     return BINE_OK;
 }
 
+extern "C" int bine_vcomm_set_cache_limit(bine_vcomm_t comm, uint32_t max_plans) {
+    if (!comm) return fail(BINE_ERR_ARG, "NULL vcomm");
+    std::lock_guard<std::mutex> lock(comm->mu);
+    (void)hipSetDevice(comm->cfg.device);
+    comm->max_plans = max_plans;
+    if (max_plans > 0) evict_lru(comm, max_plans);
+    return BINE_OK;
+}
+
+extern "C" int bine_vcomm_trim(bine_vcomm_t comm) {
+    if (!comm) return fail(BINE_ERR_ARG, "NULL vcomm");
+    std::lock_guard<std::mutex> lock(comm->mu);
+    (void)hipSetDevice(comm->cfg.device);
+    comm->plans.clear();
+    return BINE_OK;
+}
+
+extern "C" int bine_vcomm_cache_info(bine_vcomm_t comm, uint32_t* plans, uint64_t* tmp_bytes) {
+    if (!comm) return fail(BINE_ERR_ARG, "NULL vcomm");
+    std::lock_guard<std::mutex> lock(comm->mu);
+    uint64_t t = 0;
+    for (auto& kv : comm->plans)
+        for (int i = 0; i < 2; ++i)
+            if (kv.second->tmp[i]) t += kv.second->tmp_bytes[i];
+    if (plans) *plans = (uint32_t)comm->plans.size();
+    if (tmp_bytes) *tmp_bytes = t;
+    return BINE_OK;
+}
+
 // ============================================================================
 // collectives
 // ============================================================================
@@ -545,9 +703,10 @@ extern "C" int bine_plan_stats(bine_vcomm_t comm, int32_t coll, int count, bine_
                                int root, int32_t exec, bine_plan_stats_t* out) {
     if (!comm || !out) return fail(BINE_ERR_ARG, "NULL argument");
     CollRequest r{coll, count, (int)op, root};
-    DevPlan* dp = nullptr;
-    int rc = get_plan(comm, r, datatype, &dp);
+    std::shared_ptr<DevPlan> dpp;
+    int rc = get_plan(comm, r, datatype, &dpp);
     if (rc) return rc;
+    const DevPlan* dp = dpp.get();
     if (exec == BINE_EXEC_AUTO) exec = dp->cp.fused_ok ? BINE_EXEC_FUSED : BINE_EXEC_STEP;
     if (exec == BINE_EXEC_FUSED && !dp->cp.fused_ok) return fail(BINE_ERR_UNSUPPORTED, "schedule cannot be fused: " + dp->cp.fused_why);
     const DevArrays& d = exec == BINE_EXEC_FUSED ? dp->fused : dp->step;
@@ -602,9 +761,11 @@ extern "C" int bine_plan_autotune(bine_vcomm_t comm, int32_t coll, const void* s
                                   int iters) {
     if (!comm || iters < 1) return fail(BINE_ERR_ARG, "bad arguments");
     CollRequest r{coll, count, (int)op, root};
-    DevPlan* dp = nullptr;
-    int rc = get_plan(comm, r, datatype, &dp);
+    r.in_place = coll == BINE_COLL_ALLREDUCE && sendbuf && sendbuf == recvbuf;
+    std::shared_ptr<DevPlan> dpp;
+    int rc = get_plan(comm, r, datatype, &dpp);
     if (rc) return rc;
+    DevPlan* dp = dpp.get();
     if (exec == BINE_EXEC_AUTO) exec = dp->cp.fused_ok ? BINE_EXEC_FUSED : BINE_EXEC_STEP;
     if (exec == BINE_EXEC_FUSED && !dp->cp.fused_ok) return fail(BINE_ERR_UNSUPPORTED, "schedule cannot be fused");
     DevArrays& d = exec == BINE_EXEC_FUSED ? dp->fused : dp->step;
@@ -624,8 +785,12 @@ extern "C" int bine_plan_autotune(bine_vcomm_t comm, int32_t coll, const void* s
     float best = 1e30f;
     LaunchShape bestc = d.shape;
     rc = BINE_OK;
+    bool first = true;
     for (const LaunchShape& c : cands) {
-        if ((rc = upload(groups, dp->ds, c, 0, &d))) break;
+        // the arrays in place before the call may be named by captured graphs: they are kept, not freed;
+        // the candidates' own arrays never leave this loop
+        if ((rc = upload(groups, dp->ds, c, 0, &d, first ? &dp->retired : nullptr))) break;
+        first = false;
         if ((rc = issue(comm, coll, sendbuf, recvbuf, count, datatype, op, root, stream))) break;
         if (hipEventRecord(e0, st) != hipSuccess) { rc = fail(BINE_ERR_HIP, "event"); break; }
         for (int i = 0; i < iters && !rc; ++i) rc = issue(comm, coll, sendbuf, recvbuf, count, datatype, op, root, stream);
@@ -647,8 +812,8 @@ extern "C" int bine_plan_autotune(bine_vcomm_t comm, int32_t coll, const void* s
 }
 
 extern "C" int bine_plan_dump(const bine_vcomm_config_t* cfg, int32_t coll, int count, bine_datatype_t datatype,
-                              bine_op_t op, int root, int32_t exec, int64_t tile_elems, bine_plan_dump_info_t* info,
-                              void* items, void* pool, uint32_t* launches) {
+                              bine_op_t op, int root, int32_t in_place, int32_t exec, int64_t tile_elems,
+                              bine_plan_dump_info_t* info, void* items, void* pool, uint32_t* launches) {
     if (!cfg || !info) return fail(BINE_ERR_ARG, "NULL argument");
     const size_t ds = dtsize(datatype);
     if (!ds || count < 0) return fail(BINE_ERR_ARG, "bad datatype or count");
@@ -656,6 +821,7 @@ extern "C" int bine_plan_dump(const bine_vcomm_config_t* cfg, int32_t coll, int 
     Torus t;
     if (!Torus::make(cfg->dimensions, cfg->dimensions_num, &t) || t.size < 2) return fail(BINE_ERR_ARG, "bad torus");
     CollRequest r{coll, count, (int)op, root};
+    r.in_place = in_place != 0;
     Program prog;
     CompiledPlan cp;
     std::string err;
@@ -743,17 +909,39 @@ extern "C" int bine_get_tree(const uint32_t* dims, uint32_t ndims, uint32_t root
 
 extern "C" uint32_t bine_remap_rank(uint32_t rank, uint32_t size) { return remap_rank(rank, size); }
 
+namespace {
+
+// Shared checks of the two reduce_local forms: out = a + b.  out may be a or b exactly; any other overlap of
+// out with an input would race in the grid-stride kernel.  VOID leaves out as it is.
+int reduce_local_common(const void* a, const void* b, void* out, int count, bine_datatype_t datatype, bine_op_t op,
+                        void* stream) {
+    const size_t ds = dtsize(datatype);
+    if (!ds) return fail(BINE_ERR_ARG, "unknown datatype");
+    if (count < 0) return fail(BINE_ERR_ARG, "negative count");
+    if (op != BINE_OP_SUM && op != BINE_OP_VOID) return fail(BINE_ERR_ARG, "reduce_local supports SUM and VOID only");
+    if (op == BINE_OP_VOID || count == 0) return BINE_OK;
+    const size_t bytes = (size_t)count * ds;
+    int rc;
+    if ((rc = check_range(a, bytes, "inbuf"))) return rc;
+    if ((rc = check_range(b, bytes, "inbuf"))) return rc;
+    if ((rc = check_range(out, bytes, "outbuf"))) return rc;
+    if ((out != a && overlap(out, bytes, a, bytes)) || (out != b && overlap(out, bytes, b, bytes)))
+        return fail(BINE_ERR_ARG, "output partially overlaps an input (only the identical buffer is allowed)");
+    hipError_t e = launch_reduce_local(datatype, a, b, out, (uint64_t)count, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(BINE_ERR_HIP, std::string("reduce_local launch: ") + hipGetErrorString(e));
+    return BINE_OK;
+}
+
+}  // namespace
+
 extern "C" int bine_reduce_local(const void* inbuf, void* inoutbuf, int count, bine_datatype_t datatype, bine_op_t op,
                                  void* stream) {
-    const size_t ds = dtsize(datatype);
-    if (!ds || count < 0) return fail(BINE_ERR_ARG, "bad arguments");
-    if (op == BINE_OP_VOID || count == 0) return BINE_OK;
-    int rc;
-    if ((rc = check_range(inbuf, (size_t)count * ds, "inbuf"))) return rc;
-    if ((rc = check_range(inoutbuf, (size_t)count * ds, "inoutbuf"))) return rc;
-    hipError_t e = launch_reduce_local(datatype, inbuf, inoutbuf, (uint64_t)count, (hipStream_t)stream);
-    if (e != hipSuccess) return fail(BINE_ERR_HIP, hipGetErrorString(e));
-    return BINE_OK;
+    return reduce_local_common(inbuf, inoutbuf, inoutbuf, count, datatype, op, stream);
+}
+
+extern "C" int bine_reduce_local3(const void* inbuf_a, const void* inbuf_b, void* outbuf, int count,
+                                  bine_datatype_t datatype, bine_op_t op, void* stream) {
+    return reduce_local_common(inbuf_a, inbuf_b, outbuf, count, datatype, op, stream);
 }
 
 extern "C" const char* bine_last_error(void) { return g_err.c_str(); }
@@ -765,7 +953,7 @@ extern "C" const char* bine_last_error(void) { return g_err.c_str(); }
 struct bine_batch_s {
     bine_vcomm_t comm = nullptr;
     std::vector<bine_batch_item_t> items;
-    std::vector<DevPlan*> plans;
+    std::vector<std::shared_ptr<DevPlan>> plans;  // held for the batch's life (the cache may evict them)
     std::vector<int> use_fused;  // per item
     // one fused work list per element type
     struct TypeList {
@@ -793,9 +981,9 @@ struct bine_batch_s {
 
 namespace {
 
+// Lower and upload one element type's work list.  On failure the list keeps its previous arrays.
 int lower_list(bine_batch_t b, bine_batch_s::TypeList& l) {
     {
-        l.arr.release();
         const size_t ds = dtsize(l.dtype);
         // one lowering for the whole list: every work item carries its group's bucket index, runs of copies
         // form within a bucket, and items are ordered tile-major across all buckets
@@ -807,15 +995,10 @@ int lower_list(bine_batch_t b, bine_batch_s::TypeList& l) {
         int max_lg = max_lg_of(l.groups);
         bool tables_off16 = false;
         for (uint32_t gb : l.group_bucket) tables_off16 = tables_off16 || off16(b->tabs[gb]);
-        l.allow_vec = realign_pays(all, tables_off16) ? 3u : 1u;
-        l.arr.low = std::move(all);
-        l.arr.max_lg = max_lg;
-        if (!l.arr.low.items.empty()) {
-            HIPCHK(hipMalloc(&l.arr.items, l.arr.low.items.size() * sizeof(GredItem)));
-            HIPCHK(hipMemcpy(l.arr.items, l.arr.low.items.data(), l.arr.low.items.size() * sizeof(GredItem), hipMemcpyHostToDevice));
-            HIPCHK(hipMalloc(&l.arr.pool, l.arr.low.pool.size() * sizeof(PoolEntry)));
-            HIPCHK(hipMemcpy(l.arr.pool, l.arr.low.pool.data(), l.arr.low.pool.size() * sizeof(PoolEntry), hipMemcpyHostToDevice));
-        }
+        const uint32_t allow = realign_pays(all, tables_off16) ? 3u : 1u;
+        int rc = install(std::move(all), max_lg, shallow_tile_elems(l.shape, ds), l.shape, &l.arr, nullptr);
+        if (rc) return rc;
+        l.allow_vec = allow;
     }
     return BINE_OK;
 }
@@ -853,9 +1036,7 @@ int batch_run(bine_batch_t b, hipStream_t st) {
     // collectives whose schedule cannot be fused run their step launches after
     for (size_t i = 0; i < b->items.size(); ++i) {
         if (b->use_fused[i]) continue;
-        DevPlan* dp = b->plans[i];
-        BufTable tab = b->tabs[i];
-        int rc = launch_arrays(dp->step, dp->dtype, tab, vec_ok(tab), st);
+        int rc = launch_step_ordered(b->comm, b->plans[i].get(), b->tabs[i], st);
         if (rc) return rc;
     }
     return BINE_OK;
@@ -872,18 +1053,24 @@ extern "C" int bine_batch_create(bine_vcomm_t comm, const bine_batch_item_t* ite
         const bine_batch_item_t& it = items[i];
         if (it.count < 0) return fail(BINE_ERR_ARG, "negative count");
         CollRequest r{it.coll, it.count, it.op, it.root};
-        DevPlan* dp = nullptr;
-        int rc = get_plan(comm, r, it.datatype, &dp);
+        // an allreduce with sendbuf == recvbuf runs its in-place schedule per step
+        r.in_place = it.coll == BINE_COLL_ALLREDUCE && it.sendbuf && it.sendbuf == it.recvbuf;
+        std::shared_ptr<DevPlan> dpp;
+        int rc = get_plan(comm, r, it.datatype, &dpp);
         if (rc) return rc;
+        DevPlan* dp = dpp.get();
         const void* send = it.coll == BINE_COLL_BCAST ? nullptr : it.sendbuf;
+        if (r.in_place) send = nullptr;
         if ((rc = check_range(send, partition_bytes(dp, BUF_SEND), "batch sendbuf"))) return rc;
         if ((rc = check_range(it.recvbuf, partition_bytes(dp, BUF_RECV), "batch recvbuf"))) return rc;
         if (overlap(send, partition_bytes(dp, BUF_SEND), it.recvbuf, partition_bytes(dp, BUF_RECV)))
             return fail(BINE_ERR_ARG, "batch item buffers overlap");
+        if (r.in_place && comm->cfg.exec == BINE_EXEC_FUSED)
+            return fail(BINE_ERR_UNSUPPORTED, "fused execution cannot run in place (sendbuf == recvbuf)");
         const bool fuse = dp->cp.fused_ok && comm->cfg.exec != BINE_EXEC_STEP;
         if (!fuse && (rc = ensure_tmp(comm, dp))) return rc;
-        b->plans.push_back(dp);
-        b->tabs.push_back(make_table(dp, send, it.recvbuf));
+        b->plans.push_back(dpp);
+        b->tabs.push_back(make_table(dp, r.in_place ? it.recvbuf : send, it.recvbuf));
         b->use_fused.push_back(fuse ? 1 : 0);
         if (!fuse) continue;
         auto lit = std::find_if(b->lists.begin(), b->lists.end(), [&](const bine_batch_s::TypeList& l) { return l.dtype == it.datatype; });
@@ -900,8 +1087,8 @@ extern "C" int bine_batch_create(bine_vcomm_t comm, const bine_batch_item_t* ite
     for (size_t i = 0; i < b->items.size(); ++i)
         for (size_t j = 0; j < b->items.size(); ++j) {
             if (i == j) continue;
-            const DevPlan* pi = b->plans[i];
-            const DevPlan* pj = b->plans[j];
+            const DevPlan* pi = b->plans[i].get();
+            const DevPlan* pj = b->plans[j].get();
             const void* si = b->items[i].coll == BINE_COLL_BCAST ? nullptr : b->items[i].sendbuf;
             if (overlap(b->items[j].recvbuf, partition_bytes(pj, BUF_RECV), b->items[i].recvbuf, partition_bytes(pi, BUF_RECV)) ||
                 overlap(b->items[j].recvbuf, partition_bytes(pj, BUF_RECV), si, partition_bytes(pi, BUF_SEND)))

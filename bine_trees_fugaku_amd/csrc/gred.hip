@@ -484,10 +484,39 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
     }
 }
 
-template <typename T>
-__global__ void reduce_local_kernel(const T* __restrict__ in, T* __restrict__ inout, uint64_t n) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        inout[i] = add1<T>(in[i], inout[i]);
+// out = a + b element-wise (reduce_local, lib/libbine_coll.cc:153-267).  out may be a or b itself (each
+// element is read before it is written, by the same lane).  VEC: the first nvec * (16 / sizeof(T))
+// elements move as 16-byte vectors (every base dword-aligned; the global memory serves unaligned 16-byte
+// accesses), VPT vectors per lane per pass; the rest, and everything when !VEC, element by element.
+template <typename T, bool VEC, int VPT>
+__global__ void __launch_bounds__(256) reduce_local_kernel(const T* a, const T* b, T* out, uint64_t nvec,
+                                                           uint64_t n) {
+    const uint64_t lanes = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t done = 0;
+    if (VEC) {
+        const char* pa = reinterpret_cast<const char*>(a);
+        const char* pb = reinterpret_cast<const char*>(b);
+        char* po = reinterpret_cast<char*>(out);
+        for (uint64_t v0 = tid; v0 < nvec; v0 += lanes * VPT) {
+            Vec<T> x[VPT], y[VPT];
+#pragma unroll
+            for (int u = 0; u < VPT; ++u) {
+                const uint64_t v = v0 + (uint64_t)u * lanes;
+                if (v < nvec) {
+                    x[u] = vload<T, true>(pa + v * 16);
+                    y[u] = vload<T, true>(pb + v * 16);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < VPT; ++u) {
+                const uint64_t v = v0 + (uint64_t)u * lanes;
+                if (v < nvec) vstore<T, true>(po + v * 16, vadd<T>(x[u], y[u]));
+            }
+        }
+        done = nvec * (uint64_t)Vec<T>::N;
+    }
+    for (uint64_t i = done + tid; i < n; i += lanes) out[i] = add1<T>(a[i], b[i]);
 }
 
 // ---- host-side dispatch ------------------------------------------------------
@@ -587,19 +616,37 @@ hipError_t launch_grun(uint32_t esize, const GredItem* items, uint32_t n, const 
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_local(int dtype, const void* in, void* inout, uint64_t n, hipStream_t st) {
+template <typename T>
+static hipError_t launch_rl(const void* a, const void* b, void* out, uint64_t n, hipStream_t st) {
+    const bool vec = (((uintptr_t)a | (uintptr_t)b | (uintptr_t)out) & 3u) == 0;
+    const uint64_t nvec = vec ? n / (uint64_t)Vec<T>::N : 0;
+    constexpr int kVpt = 4;
+    // enough lanes for every vector to be one lane's within one pass of VPT, capped at 8 workgroups per CU
+    uint64_t work = vec ? (nvec + kVpt - 1) / kVpt : n;
+    if (work == 0) work = n;  // only a scalar tail
+    uint64_t blocks = (work + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks == 0) blocks = 1;
+    if (vec)
+        hipLaunchKernelGGL((reduce_local_kernel<T, true, kVpt>), dim3(blocks), dim3(256), 0, st, (const T*)a,
+                           (const T*)b, (T*)out, nvec, n);
+    else
+        hipLaunchKernelGGL((reduce_local_kernel<T, false, kVpt>), dim3(blocks), dim3(256), 0, st, (const T*)a,
+                           (const T*)b, (T*)out, nvec, n);
+    return hipGetLastError();
+}
+
+// out = a + b over n elements of bine_datatype_t dtype
+hipError_t launch_reduce_local(int dtype, const void* a, const void* b, void* out, uint64_t n, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    uint64_t blocks = (n + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
     switch (dtype) {
-    case 0: hipLaunchKernelGGL(reduce_local_kernel<int32_t>, dim3(blocks), dim3(256), 0, st, (const int32_t*)in, (int32_t*)inout, n); break;
-    case 1: hipLaunchKernelGGL(reduce_local_kernel<int64_t>, dim3(blocks), dim3(256), 0, st, (const int64_t*)in, (int64_t*)inout, n); break;
-    case 2: hipLaunchKernelGGL(reduce_local_kernel<float>, dim3(blocks), dim3(256), 0, st, (const float*)in, (float*)inout, n); break;
-    case 3: hipLaunchKernelGGL(reduce_local_kernel<double>, dim3(blocks), dim3(256), 0, st, (const double*)in, (double*)inout, n); break;
-    case 4: hipLaunchKernelGGL(reduce_local_kernel<int8_t>, dim3(blocks), dim3(256), 0, st, (const int8_t*)in, (int8_t*)inout, n); break;
+    case 0: return launch_rl<int32_t>(a, b, out, n, st);
+    case 1: return launch_rl<int64_t>(a, b, out, n, st);
+    case 2: return launch_rl<float>(a, b, out, n, st);
+    case 3: return launch_rl<double>(a, b, out, n, st);
+    case 4: return launch_rl<int8_t>(a, b, out, n, st);
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 #endif  // BINE_GRED_REALIGN

@@ -261,7 +261,11 @@ bool allreduce_b_cont(const Torus& t, int ports, int family, int64_t count, int 
 // bine_coll_l_mpi with shrink / enlarge for non power-of-two dimensions
 // (lib/libbine_common.cc:733-846, 269-353).  Each rank's running value moves
 // between RECV and TMP so that a step never overwrites what a peer reads.
-bool allreduce_l(const Torus& t, int ports, int family, int64_t count, int op, Program* p, std::string* err) {
+// In place, the running value starts in RECV instead of SEND, so the first
+// step writes TMP (a peer still reads RECV) and an odd step count ends with a
+// copy back into RECV.
+bool allreduce_l(const Torus& t, int ports, int family, int64_t count, int op, bool in_place, Program* p,
+                 std::string* err) {
     const uint32_t P = t.size, D = t.nd;
     p->extent[BUF_SEND] = p->extent[BUF_RECV] = p->extent[BUF_TMP] = count;
     uint32_t vd[kMaxDims] = {1, 1, 1};
@@ -280,7 +284,7 @@ bool allreduce_l(const Torus& t, int ports, int family, int64_t count, int op, P
         return false;
     }
     PB b{p, op};
-    std::vector<uint8_t> cur(P, BUF_SEND);
+    std::vector<uint8_t> cur(P, in_place ? BUF_RECV : BUF_SEND);
     std::vector<char> idle(P, 0);
     if (!all_p2) {
         for (uint32_t i = 0; i < D; ++i) {
@@ -1139,6 +1143,31 @@ bool gather_tree(uint32_t P, int64_t c, int root, Program* p, std::string* err) 
     return true;
 }
 
+// MPI_IN_PLACE: SEND and RECV are one buffer.  Every SEND location becomes the
+// same RECV location and copies that turn into identities are dropped (the
+// initial SEND -> RECV copy of B).  The symbolic race check then runs on the
+// aliased program, so a schedule that reads a SEND partition after another
+// operation of the step wrote that RECV partition is rejected.
+void alias_send_onto_recv(Program* p) {
+    auto fix = [](Loc& l) {
+        if (l.buf == BUF_SEND) l.buf = BUF_RECV;
+    };
+    for (auto& st : p->steps) {
+        std::vector<Op> kept;
+        kept.reserve(st.size());
+        for (Op o : st) {
+            fix(o.dst);
+            fix(o.a);
+            fix(o.b);
+            if (o.kind == OP_COPY && o.dst.buf == o.a.buf && o.dst.rank == o.a.rank && o.dst.off == o.a.off) continue;
+            kept.push_back(o);
+        }
+        st.swap(kept);
+    }
+    p->extent[BUF_SEND] = 0;
+    if (p->expect.src_buf == BUF_SEND) p->expect.src_buf = BUF_RECV;
+}
+
 }  // namespace
 
 bool build_program(const Torus& t, const bine_vcomm_config_t& cfg, const CollRequest& req, Program* p,
@@ -1151,6 +1180,10 @@ bool build_program(const Torus& t, const bine_vcomm_config_t& cfg, const CollReq
         *err = "negative count";
         return false;
     }
+    if (req.in_place && req.coll != BINE_COLL_ALLREDUCE) {
+        *err = "only allreduce runs in place (sendbuf == recvbuf)";
+        return false;
+    }
     if (req.op != BINE_OP_SUM && req.op != BINE_OP_VOID) {
         *err = "unknown op";
         return false;
@@ -1160,20 +1193,25 @@ bool build_program(const Torus& t, const bine_vcomm_config_t& cfg, const CollReq
     case BINE_COLL_ALLREDUCE: {
         const bine_coll_config_t& c = cfg.allreduce;
         if (!fam_ok(c.family)) { *err = "allreduce family must be BINE or RECDOUB"; return false; }
+        bool ok;
         switch (c.algo) {
         case BINE_ALLREDUCE_L:
-            return allreduce_l(t, ports, c.family, req.count, req.op, p, err);
+            ok = allreduce_l(t, ports, c.family, req.count, req.op, req.in_place, p, err);
+            break;
         case BINE_ALLREDUCE_B:
         case BINE_ALLREDUCE_B_COALESCE:
         case BINE_ALLREDUCE_B_CONT:
             if (c.distance != BINE_DISTANCE_INCREASING) { *err = "allreduce B needs INCREASING distance (lib/libbine.cc:1035)"; return false; }
             if (req.count < (int64_t)ports * P) { *err = "count < num_ports * P (lib/libbine.cc:1036)"; return false; }
-            if (c.algo == BINE_ALLREDUCE_B_CONT) return allreduce_b_cont(t, ports, c.family, req.count, req.op, p, err);
-            return allreduce_b(t, ports, c.family, req.count, req.op, p, err);
+            ok = c.algo == BINE_ALLREDUCE_B_CONT ? allreduce_b_cont(t, ports, c.family, req.count, req.op, p, err)
+                                                 : allreduce_b(t, ports, c.family, req.count, req.op, p, err);
+            break;
         default:
             *err = "allreduce algorithm not available in the reference MPI layer";
             return false;
         }
+        if (ok && req.in_place) alias_send_onto_recv(p);
+        return ok;
     }
     case BINE_COLL_REDUCE_SCATTER: {
         const bine_coll_config_t& c = cfg.reduce_scatter;

@@ -92,9 +92,12 @@ struct CollRequest {
     int64_t count = 0;  // per-rank count as in the MPI call (block count for RS/AG/A2A/scatter/gather)
     int op = BINE_OP_SUM;
     int root = 0;
+    bool in_place = false;  // allreduce with sendbuf == recvbuf: the input starts in RECV
 };
 
 // Build the step program of one collective for the given vcomm configuration.
+// An in-place request gets a program over RECV alone (SEND has no extent), so
+// the symbolic race check sees the aliasing.
 bool build_program(const Torus& t, const bine_vcomm_config_t& cfg, const CollRequest& req, Program* out,
                    std::string* err);
 

@@ -22,8 +22,23 @@
  * reference asserts (lib/libbine.cc:1035-1036, 1085, 148, lib/libbine_coll.cc:204)
  * are reported as BINE_ERR_ARG / BINE_ERR_UNSUPPORTED instead of aborting.
  * Schedules are compiled on the first call for a given (collective, count,
- * type, op, root) and cached in the vcomm; a call with a cached schedule does
- * not allocate and may be captured into a HIP graph.
+ * type, op, root, in place or not) and cached in the vcomm; a call with a
+ * cached schedule does not allocate (a per-step run also needs its
+ * temporaries, allocated on its first run) and may be captured into a HIP
+ * graph.  Such a graph stays valid while that schedule stays cached: the
+ * cache keeps the 4096 most recently used schedules (see
+ * bine_vcomm_set_cache_limit), and bine_vcomm_trim empties it; batches hold
+ * their own schedules.
+ *
+ * MPI_IN_PLACE: bine_allreduce with sendbuf == recvbuf reads the input from
+ * recvbuf.  It runs the per-step schedule compiled for that aliasing (every
+ * algorithm: B and B_CONT drop their initial copy, L keeps its running value
+ * off RECV at the first step); other collectives reject overlapping buffers.
+ *
+ * Per-step runs of one schedule share its temporaries.  Outside graph
+ * capture, each such run waits on the GPU for the previous one, whatever
+ * streams they were issued on; calls captured into graphs must be ordered by
+ * the caller.
  */
 #ifndef BINE_HIP_H
 #define BINE_HIP_H
@@ -111,6 +126,16 @@ int bine_vcomm_destroy(bine_vcomm_t comm);
 int bine_vcomm_size(bine_vcomm_t comm, uint32_t* size);
 /* Replace the execution mode of an existing vcomm (cached schedules are kept). */
 int bine_vcomm_set_exec(bine_vcomm_t comm, int32_t exec);
+/* Plan cache (no counterpart in the reference, whose bitmap and tree caches are never evicted,
+ * lib/libbine_common.cc:1671-1675, lib/libbine_coll.cc:334-389).  At most max_plans schedules stay
+ * cached, least recently used evicted first (0 = no bound; the default is 4096).  When a per-step
+ * run cannot allocate its temporaries, the temporaries of schedules that only the cache holds are
+ * released, least recently used first, and the allocation is retried. */
+int bine_vcomm_set_cache_limit(bine_vcomm_t comm, uint32_t max_plans);
+/* Drop every cached schedule (batches keep theirs). */
+int bine_vcomm_trim(bine_vcomm_t comm);
+/* Schedules cached and bytes of per-step temporaries they hold. */
+int bine_vcomm_cache_info(bine_vcomm_t comm, uint32_t* plans, uint64_t* tmp_bytes);
 
 /* MPI_Allreduce, lib/libbine.cc:1012 */
 int bine_allreduce(const void* sendbuf, void* recvbuf, int count, bine_datatype_t datatype, bine_op_t op,
@@ -150,9 +175,16 @@ int bine_get_tree(const uint32_t* dims, uint32_t ndims, uint32_t root, uint32_t 
                   uint32_t* remapped_ranks_max, uint32_t* subtree_roots);
 /* remap_rank, lib/libbine_coll_alltoall.cc:372 */
 uint32_t bine_remap_rank(uint32_t rank, uint32_t size);
-/* reduce_local, lib/libbine_coll.cc:153 (2-argument form: inout = in + inout), on device */
+/* reduce_local, lib/libbine_coll.cc:153-207 (2-argument form: inout = in + inout), on device.
+ * SUM, or VOID (no effect, as the reference's INT32 branch treats a non-SUM op); any other op is
+ * BINE_ERR_ARG where the reference exits.  All five element types (the reference has no
+ * int64/double).  inbuf may be inoutbuf itself; a partial overlap is BINE_ERR_ARG. */
 int bine_reduce_local(const void* inbuf, void* inoutbuf, int count, bine_datatype_t datatype, bine_op_t op,
                       void* stream);
+/* reduce_local, lib/libbine_coll.cc:209-267 (3-argument form: out = a + b), on device.  VOID writes
+ * nothing.  outbuf may be either input itself; a partial overlap is BINE_ERR_ARG. */
+int bine_reduce_local3(const void* inbuf_a, const void* inbuf_b, void* outbuf, int count,
+                       bine_datatype_t datatype, bine_op_t op, void* stream);
 
 /* ---- introspection: bytes a call moves (for roofline accounting) ---- */
 typedef struct {
@@ -174,8 +206,12 @@ int bine_plan_stats(bine_vcomm_t comm, int32_t coll, int count, bine_datatype_t 
                     int32_t exec, bine_plan_stats_t* out);
 
 /* Time candidate launch shapes (vectors per thread, workgroup size, persistent
- * grid) of one collective's cached schedule on the caller's buffers and keep
- * the fastest for later calls.  Runs the collective 13*(iters+1) times. */
+ * grid, streaming) of one collective's cached schedule on the caller's buffers
+ * and keep the fastest for later calls.  Runs the collective (iters + 1) times
+ * per candidate (bine_tuning_candidates lists them; 28 on an MI355X).  The
+ * work list in place before the call stays allocated until the vcomm is
+ * destroyed, so graphs captured earlier keep running the old launch shape.
+ * Must not run concurrently with other calls on the same vcomm. */
 int bine_plan_autotune(bine_vcomm_t comm, int32_t coll, const void* sendbuf, void* recvbuf, int count,
                        bine_datatype_t datatype, bine_op_t op, int root, int32_t exec, void* stream, int iters);
 
@@ -183,7 +219,8 @@ int bine_plan_autotune(bine_vcomm_t comm, int32_t coll, const void* sendbuf, voi
  * lowered work (the arrays one call would upload).  Call once with NULL
  * arrays to size them.  items: info->n_items records of 32 bytes (bine
  * gather-reduce items); pool: info->n_pool records of 8 bytes; launches:
- * info->n_launches pairs (first item, item count).  No HIP call is made. */
+ * info->n_launches pairs (first item, item count).  in_place: the allreduce
+ * schedule for sendbuf == recvbuf (SEND has no extent).  No HIP call is made. */
 typedef struct {
     uint64_t n_items, n_pool, n_launches;
     int64_t extent[4];  /* per-rank elements of SEND, RECV, TMP, TMP2 */
@@ -192,8 +229,8 @@ typedef struct {
     int32_t fused_ok;
 } bine_plan_dump_info_t;
 int bine_plan_dump(const bine_vcomm_config_t* cfg, int32_t coll, int count, bine_datatype_t datatype, bine_op_t op,
-                   int root, int32_t exec, int64_t tile_elems, bine_plan_dump_info_t* info, void* items, void* pool,
-                   uint32_t* launches);
+                   int root, int32_t in_place, int32_t exec, int64_t tile_elems, bine_plan_dump_info_t* info,
+                   void* items, void* pool, uint32_t* launches);
 
 /* ---- batching: many collectives in one launch (no counterpart in the reference:
  * it replaces the per-call read_env/BlockInfo/tmpbuf set-up, lib/libbine.cc:1013-1050,
@@ -212,7 +249,10 @@ typedef struct bine_batch_s* bine_batch_t;
 
 /* Compile `n` collectives on one vcomm into one fused work list.  The buffers
  * are bound here and must stay allocated, and the vcomm alive, until
- * bine_batch_destroy. */
+ * bine_batch_destroy.  An allreduce item with sendbuf == recvbuf runs in place,
+ * per step, after the fused lists.  Retuning a batch (set_tuning, autotune)
+ * replaces its work lists: graphs the caller captured from bine_batch_launch
+ * before that must be captured again (bine_batch_launch_graph does so itself). */
 int bine_batch_create(bine_vcomm_t comm, const bine_batch_item_t* items, int n, bine_batch_t* out);
 /* Run the whole batch: one persistent launch per element type present. */
 int bine_batch_launch(bine_batch_t batch, void* stream);

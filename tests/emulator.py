@@ -34,7 +34,9 @@ def alloc_buffers(meta, dtype, send=None, recv=None):
     return bufs
 
 
-def run(meta, items, pool, launches, bufs):
+def run(meta, items, pool, launches, bufs, reverse=False):
+    """reverse: run each launch's items last to first (a launch's result must not
+    depend on the order its items run in)."""
     single = meta["single"]
 
     def view(rb, off, n):
@@ -45,7 +47,8 @@ def run(meta, items, pool, launches, bufs):
 
     with np.errstate(over="ignore"):
         for first, cnt in launches:
-            for it in items[first:first + cnt]:
+            launch = items[first:first + cnt]
+            for it in (launch[::-1] if reverse else launch):
                 n = int(it["n"])
                 d = int(it["delta"])
                 if int(it["flags"]) & ITEM_RUN:  # k copies advancing by a fixed (rank, offset) step

@@ -283,3 +283,55 @@ def test_long_copies_lower_to_tile_pairs(coll, count):
     else:
         _, bufs = run_plan(dims, coll, count, "int32", "FUSED", send=x, tile=tile, root=0)
         assert same_bits(bufs[E.BUF_RECV][0], O.gather(x, 0))
+
+
+# ---- MPI_IN_PLACE allreduce ------------------------------------------------------------
+
+IN_PLACE_TORI = [[8], [2, 2, 2], [2, 8, 2], [8, 8, 8], [4, 4], [16], [6], [10], [6, 6], [2, 6, 2]]
+
+
+@pytest.mark.parametrize("dims", IN_PLACE_TORI)
+@pytest.mark.parametrize("algo", ["B", "B_COALESCE", "L", "B_CONT"])
+@pytest.mark.parametrize("ports", [1, 3])
+def test_allreduce_in_place(dims, algo, ports):
+    """sendbuf == recvbuf: the in-place schedule reads RECV only and gives the oracle's bits
+    whatever order a launch's items run in (odd step counts included: [8], [2, 2, 2], [8, 8, 8])."""
+    P = int(np.prod(dims))
+    if algo == "B_CONT" and P & (P - 1):
+        pytest.skip("B_CONT is power-of-two only")
+    if ports > 1 and (len(dims) < 3 or P > 64):
+        pytest.skip("ports > 1 on the small 3D tori only")
+    count = 3 * ports * P + 5
+    x = data((P, count), "float64", P + ports)
+    ref = O.allreduce(x, dims, algo=algo, ports=ports)
+    meta, items, pool, launches = plan_dump(dims, "allreduce", count, "float64", exec="STEP", in_place=True,
+                                            num_ports=ports, allreduce_algo=algo, tile_elems=1 << 16)
+    assert meta["extent"][0] == 0 and not meta["fused_ok"]
+    for rev in (False, True):
+        bufs = E.alloc_buffers(meta, np.float64, recv=x)
+        E.run(meta, items, pool, launches, bufs, reverse=rev)
+        assert same_bits(bufs[E.BUF_RECV], ref), (algo, rev)
+
+
+def test_allreduce_in_place_needs_its_own_schedule():
+    """The out-of-place L schedule on an odd step count writes RECV at its first step while
+    other ranks still read SEND: run with SEND and RECV as one array, it goes wrong in at
+    least one item order.  That is why in-place calls get a schedule of their own."""
+    dims, P = [8], 8
+    count = 3 * P + 5
+    x = data((P, count), "float64", 3)
+    ref = O.allreduce(x, dims, algo="L")
+    meta, items, pool, launches = plan_dump(dims, "allreduce", count, "float64", exec="STEP", allreduce_algo="L",
+                                            tile_elems=1 << 16)
+    wrong = 0
+    for rev in (False, True):
+        bufs = E.alloc_buffers(meta, np.float64, send=x)
+        bufs[E.BUF_RECV] = bufs[E.BUF_SEND]  # one array: MPI_IN_PLACE through the out-of-place schedule
+        E.run(meta, items, pool, launches, bufs, reverse=rev)
+        wrong += not same_bits(bufs[E.BUF_RECV], ref)
+    assert wrong >= 1
+
+
+def test_in_place_rejected_for_other_collectives():
+    with pytest.raises(BineError):
+        plan_dump([4, 4], "reduce_scatter", 4, "int32", in_place=True)

@@ -19,7 +19,7 @@ STATUS_NAMES = {0: "BINE_OK", 1: "BINE_ERR_ARG", 2: "BINE_ERR_UNSUPPORTED", 3: "
 EXPORTED = [
     "bine_vcomm_config_init", "bine_vcomm_config_from_env", "bine_vcomm_create", "bine_vcomm_destroy",
     "bine_vcomm_size", "bine_vcomm_set_exec", "bine_vcomm_set_cache_limit", "bine_vcomm_trim",
-    "bine_vcomm_cache_info", "bine_allreduce", "bine_reduce_scatter", "bine_allgather",
+    "bine_vcomm_cache_info", "bine_allreduce", "bine_allreduce_noalloc", "bine_reduce_scatter", "bine_allgather",
     "bine_bcast", "bine_alltoall", "bine_scatter", "bine_gather", "bine_reduce", "bine_compute_peers",
     "bine_compute_step_info", "bine_block_step", "bine_get_tree", "bine_remap_rank", "bine_reduce_local",
     "bine_reduce_local3",    "bine_plan_stats", "bine_plan_autotune", "bine_plan_dump", "bine_batch_create", "bine_batch_launch", "bine_batch_launch_graph",
@@ -89,6 +89,7 @@ def _sig(L):
     L.bine_vcomm_trim.argtypes = [vp]
     L.bine_vcomm_cache_info.argtypes = [vp, pu32, ctypes.POINTER(ctypes.c_uint64)]
     L.bine_allreduce.argtypes = [vp, vp, i32, i32, i32, vp, vp]
+    L.bine_allreduce_noalloc.argtypes = [vp, vp, i32, i32, i32, vp, vp]
     L.bine_reduce_scatter.argtypes = [vp, vp, pint, i32, i32, vp, vp]
     L.bine_allgather.argtypes = [vp, i32, i32, vp, i32, i32, vp, vp]
     L.bine_bcast.argtypes = [vp, i32, i32, i32, vp, vp]

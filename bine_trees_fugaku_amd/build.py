@@ -36,6 +36,7 @@ SOURCES = [
     ("gred_realign.hip", "hip"),
     ("sim.hip", "hip"),
     ("mlp.hip", "hip"),
+    ("scratch.hip", "hip"),
 ]
 
 

@@ -28,6 +28,9 @@ hipError_t launch_gred(int dtype, int max_lg, const GredItem* items, uint32_t n,
 hipError_t launch_grun(uint32_t esize, const GredItem* items, uint32_t n, const PoolEntry* pool, const BufTable& tab,
                        const BufTable* tabs, bool batch, uint32_t allow_vec, const LaunchShape& sh, hipStream_t st);
 hipError_t launch_reduce_local(int dtype, const void* a, const void* b, void* out, uint64_t n, hipStream_t st);
+size_t scratch_bytes();
+hipError_t scratch_init(int device);
+void* scratch_ptr(int device);
 }  // namespace bine
 
 using namespace bine;
@@ -201,6 +204,7 @@ struct bine_vcomm_s {
     std::unordered_map<PlanKey, std::shared_ptr<DevPlan>, PlanKeyHash> plans;
     uint32_t max_plans = kDefaultMaxPlans;
     uint64_t clock = 0;
+    hipError_t scratch_err = hipSuccess;  // resolving the allocation-free entry point's workspace at creation
 };
 
 namespace {
@@ -460,6 +464,18 @@ int run_coll(bine_vcomm_t comm, const CollRequest& req0, int dtype, const void* 
     return launch_arrays(dp->fused, dtype, tab, vec_ok(tab), (hipStream_t)stream);
 }
 
+// A cached schedule, or nullptr; never compiles (the allocation-free entry point uses it).
+std::shared_ptr<DevPlan> find_plan(bine_vcomm_t comm, const CollRequest& req, int dtype) {
+    PlanKey key{req.coll, dtype, req.op, req.root, req.count, req.in_place};
+    std::lock_guard<std::mutex> lock(comm->mu);
+    auto it = comm->plans.find(key);
+    if (it == comm->plans.end()) return nullptr;
+    it->second->last_use = ++comm->clock;
+    return it->second;
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
 int parse_family(const char* s, int* out) {
     if (!strcmp(s, "DEFAULT")) *out = BINE_FAMILY_DEFAULT;
     else if (!strcmp(s, "BINE")) *out = BINE_FAMILY_BINE;
@@ -580,6 +596,8 @@ extern "C" int bine_vcomm_create(const bine_vcomm_config_t* cfg, bine_vcomm_t* o
     auto* c = new bine_vcomm_s();
     c->cfg = *cfg;
     c->t = t;
+    // set-up of bine_allreduce_noalloc: the device address of its workspace, once per device
+    c->scratch_err = scratch_init(cfg->device);
     *out = c;
     return BINE_OK;
 }
@@ -641,6 +659,52 @@ extern "C" int bine_allreduce(const void* sendbuf, void* recvbuf, int count, bin
     if (count < 0) return fail(BINE_ERR_ARG, "negative count");
     CollRequest r{BINE_COLL_ALLREDUCE, count, (int)op, 0};
     return run_coll(comm, r, datatype, sendbuf, recvbuf, stream);
+}
+
+extern "C" int bine_allreduce_noalloc(const void* sendbuf, void* recvbuf, int count, bine_datatype_t datatype,
+                                      bine_op_t op, bine_vcomm_t comm, void* stream) {
+    if (!comm) return fail(BINE_ERR_ARG, "NULL vcomm");
+    if (count < 0) return fail(BINE_ERR_ARG, "negative count");
+    const size_t ds = dtsize(datatype);
+    if (!ds) return fail(BINE_ERR_ARG, "unknown datatype");
+    const size_t bytes = (size_t)count * ds * comm->t.size;
+    CollRequest req{BINE_COLL_ALLREDUCE, count, (int)op, 0};
+    if (overlap(sendbuf, bytes, recvbuf, bytes)) {
+        if (sendbuf != recvbuf) return fail(BINE_ERR_ARG, "sendbuf and recvbuf overlap");
+        req.in_place = true;
+    }
+    std::shared_ptr<DevPlan> dpp = find_plan(comm, req, datatype);
+    if (!dpp)
+        return fail(BINE_ERR_UNSUPPORTED, "no cached schedule for this allreduce: compile it at set-up with a "
+                                          "bine_allreduce call of the same count, type, op and in-place-ness");
+    DevPlan* dp = dpp.get();
+    int rc;
+    if ((rc = check_range(req.in_place ? nullptr : sendbuf, req.in_place ? 0 : bytes, "sendbuf"))) return rc;
+    if ((rc = check_range(recvbuf, bytes, "recvbuf"))) return rc;
+    int exec = comm->cfg.exec;
+    if (req.in_place && exec == BINE_EXEC_FUSED)
+        return fail(BINE_ERR_UNSUPPORTED, "fused execution cannot run in place (sendbuf == recvbuf)");
+    if (exec == BINE_EXEC_FUSED && !dp->cp.fused_ok)
+        return fail(BINE_ERR_UNSUPPORTED, "schedule cannot be fused: " + dp->cp.fused_why);
+    if (exec == BINE_EXEC_AUTO) exec = dp->cp.fused_ok ? BINE_EXEC_FUSED : BINE_EXEC_STEP;
+    HIPCHK(hipSetDevice(comm->cfg.device));
+    BufTable tab = make_table(dp, req.in_place ? recvbuf : sendbuf, recvbuf);
+    if (exec == BINE_EXEC_STEP && dp->tmp_total()) {
+        // the per-step temporaries come from the device's scratch array instead of the plan's allocations
+        const size_t need = align256(dp->tmp_bytes[0]) + dp->tmp_bytes[1];
+        if (need > scratch_bytes())
+            return fail(BINE_ERR_ARG, "this schedule needs " + std::to_string(need) + " bytes of per-step temporaries; "
+                                      "the allocation-free entry point holds at most " +
+                                          std::to_string(scratch_bytes()) + " bytes");
+        char* ptr = (char*)scratch_ptr(comm->cfg.device);
+        if (!ptr)
+            return fail(BINE_ERR_HIP, std::string("scratch workspace unavailable: hipGetSymbolAddress: ") +
+                                          hipGetErrorString(comm->scratch_err));
+        tab.base[BUF_TMP] = dp->tmp_bytes[0] ? (uint64_t)(uintptr_t)ptr : 0;
+        tab.base[BUF_TMP2] = dp->tmp_bytes[1] ? (uint64_t)(uintptr_t)(ptr + align256(dp->tmp_bytes[0])) : 0;
+    }
+    return launch_arrays(exec == BINE_EXEC_FUSED ? dp->fused : dp->step, datatype, tab, vec_ok(tab),
+                         (hipStream_t)stream);
 }
 
 extern "C" int bine_reduce_scatter(const void* sendbuf, void* recvbuf, const int recvcounts[], bine_datatype_t datatype,

@@ -140,6 +140,17 @@ int bine_vcomm_cache_info(bine_vcomm_t comm, uint32_t* plans, uint64_t* tmp_byte
 /* MPI_Allreduce, lib/libbine.cc:1012 */
 int bine_allreduce(const void* sendbuf, void* recvbuf, int count, bine_datatype_t datatype, bine_op_t op,
                    bine_vcomm_t comm, void* stream);
+/* bine_allreduce for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * arguments and results as bine_allreduce, in place included, but it never compiles, allocates
+ * or frees: the schedule must already be cached (compile it at set-up with one bine_allreduce
+ * call of the same count, type, op and in-place-ness; otherwise BINE_ERR_UNSUPPORTED), and a
+ * per-step run takes its temporaries from a 64 MiB device array the library holds per GPU until
+ * the process ends (a larger need is BINE_ERR_ARG).  Every call on one GPU shares that array and
+ * leaves its contents behind (the schedules write it before reading it); calls that overlap in
+ * time on one GPU race, so issue them on one stream or order them. */
+int bine_allreduce_noalloc(const void* sendbuf, void* recvbuf, int count, bine_datatype_t datatype, bine_op_t op,
+                           bine_vcomm_t comm, void* stream);
 /* MPI_Reduce_scatter, lib/libbine.cc:1064 (equal recvcounts, as the reference assumes :1065) */
 int bine_reduce_scatter(const void* sendbuf, void* recvbuf, const int recvcounts[], bine_datatype_t datatype,
                         bine_op_t op, bine_vcomm_t comm, void* stream);

@@ -75,12 +75,6 @@ def build(verbose: bool = False) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
-    probe = os.path.join(CSRC, "hbm_probe.hip")
-    if os.path.exists(probe):
-        cmd = [_hipcc(), *HIP_FLAGS, "-o", os.path.join(PKG, "hbm_probe"), probe]
-        res = subprocess.run(cmd, capture_output=True, text=True)
-        if res.returncode != 0:
-            raise RuntimeError(f"probe build failed: {' '.join(cmd)}\n{res.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -26,6 +26,44 @@ namespace bine {
 
 typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 
+// Address-space views.  Data goes through global (address space 1) pointers, so loads and stores issue
+// as global_* instructions, which count against the vector-memory counter only; generic pointers would
+// give flat_* instructions, which also hold the scalar/LDS counter, so that every wait for a scalar load
+// would drain the data loads in flight as well.  Work items, pool entries and buffer tables are read
+// through constant (address space 4) pointers with a wave-uniform address, so they issue as scalar loads
+// (one s_load_dwordx8 for four consecutive pool entries) that never wait on the data loads.
+#define BINE_GLOBAL __attribute__((address_space(1)))
+#define BINE_CONST __attribute__((address_space(4)))
+typedef BINE_GLOBAL u4 gu4;
+
+template <typename X>
+__device__ __forceinline__ BINE_GLOBAL X* gptr(const void* p) {
+    return (BINE_GLOBAL X*)(uintptr_t)p;
+}
+
+// p must be the same for every lane of the wave
+template <typename X>
+__device__ __forceinline__ const BINE_CONST X* cptr(const X* p) {
+    const uint64_t a = (uint64_t)(uintptr_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    return (const BINE_CONST X*)(((uint64_t)hi << 32) | lo);
+}
+
+// A struct read word by word through a constant pointer (a plain copy would need a copy constructor that
+// takes the generic address space).
+template <typename X>
+__device__ __forceinline__ X cload(const BINE_CONST X* p) {
+    static_assert(sizeof(X) % 4 == 0, "struct of whole words");
+    const BINE_CONST uint32_t* q = (const BINE_CONST uint32_t*)p;
+    uint32_t w[sizeof(X) / 4];
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(X) / 4; ++i) w[i] = q[i];
+    X r;
+    __builtin_memcpy(&r, w, sizeof(X));
+    return r;
+}
+
 // 16 bytes of elements of type T, held as one 128-bit register quad.
 template <typename T>
 struct Vec {
@@ -76,15 +114,15 @@ __device__ __forceinline__ Vec<int8_t> vadd<int8_t>(const Vec<int8_t>& a, const 
 template <typename T, bool NT>
 __device__ __forceinline__ Vec<T> vload(const char* p) {
     Vec<T> r;
-    if (NT) r.raw = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p));
-    else r.raw = *reinterpret_cast<const u4*>(p);
+    if (NT) r.raw = __builtin_nontemporal_load(gptr<const u4>(p));
+    else r.raw = *gptr<const u4>(p);
     return r;
 }
 
 template <typename T, bool NT>
 __device__ __forceinline__ void vstore(char* p, const Vec<T>& v) {
-    if (NT) __builtin_nontemporal_store(v.raw, reinterpret_cast<u4*>(p));
-    else *reinterpret_cast<u4*>(p) = v.raw;
+    if (NT) __builtin_nontemporal_store(v.raw, gptr<u4>(p));
+    else *gptr<u4>(p) = v.raw;
 }
 
 // ---- realigned stores ----------------------------------------------------------
@@ -118,20 +156,20 @@ __device__ __forceinline__ Prev rotate_prev(const u4& v, uint32_t lane) {
 
 template <bool NT>
 __device__ __forceinline__ void st_word(char* p, unsigned v) {
-    if (NT) __builtin_nontemporal_store(v, reinterpret_cast<unsigned*>(p));
-    else *reinterpret_cast<unsigned*>(p) = v;
+    if (NT) __builtin_nontemporal_store(v, gptr<unsigned>(p));
+    else *gptr<unsigned>(p) = v;
 }
 
 template <bool NT>
 __device__ __forceinline__ void st_pair(char* p, const u2& v) {  // p 8-byte aligned
-    if (NT) __builtin_nontemporal_store(v, reinterpret_cast<u2*>(p));
-    else *reinterpret_cast<u2*>(p) = v;
+    if (NT) __builtin_nontemporal_store(v, gptr<u2>(p));
+    else *gptr<u2>(p) = v;
 }
 
 template <bool NT>
 __device__ __forceinline__ void st_quad(char* p, const u4& v) {
-    if (NT) __builtin_nontemporal_store(v, reinterpret_cast<u4*>(p));
-    else *reinterpret_cast<u4*>(p) = v;
+    if (NT) __builtin_nontemporal_store(v, gptr<u4>(p));
+    else *gptr<u4>(p) = v;
 }
 
 // p: this lane's vector address, k = (p mod 16) / 4 in 1..3 (the same for the
@@ -247,11 +285,18 @@ __global__ void __launch_bounds__(512) grun_kernel(const GredItem* __restrict__ 
                                                    const BufTable* __restrict__ tabs, uint32_t allow_vec,
                                                    uint32_t esize) {
     for (uint32_t ii = blockIdx.x; ii < nitems; ii += gridDim.x) {
-        const GredItem it = items[ii];
-        const BufTable& bt = BATCH ? tabs[it.bucket] : tab;
-        const Table t{bt.base[0], bt.base[1], bt.base[2], bt.base[3], bt.stride[0], bt.stride[1], bt.stride[2], bt.stride[3]};
-        const PoolEntry l0 = pool[it.leaf_off], ls = pool[it.leaf_off + 1];
-        const PoolEntry o0 = pool[it.out_off], os = pool[it.out_off + 1];
+        const GredItem it = cload(cptr(items + ii));
+        Table t;
+        if (BATCH) {
+            const BINE_CONST BufTable* bt = cptr(tabs + it.bucket);
+            t = Table{bt->base[0], bt->base[1], bt->base[2], bt->base[3], bt->stride[0], bt->stride[1], bt->stride[2], bt->stride[3]};
+        } else {
+            t = Table{tab.base[0], tab.base[1], tab.base[2], tab.base[3], tab.stride[0], tab.stride[1], tab.stride[2], tab.stride[3]};
+        }
+        const BINE_CONST PoolEntry* lp = cptr(pool + it.leaf_off);
+        const BINE_CONST PoolEntry* op = cptr(pool + it.out_off);
+        const PoolEntry l0 = cload(lp), ls = cload(lp + 1);
+        const PoolEntry o0 = cload(op), os = cload(op + 1);
         const uint64_t delta = it.delta;
         const bool vec = (it.flags & ITEM_VEC) && (allow_vec & 1u);
         // two copies per pass, so every thread keeps 2 * VPT 16-byte loads in flight
@@ -269,8 +314,8 @@ __global__ void __launch_bounds__(512) grun_kernel(const GredItem* __restrict__ 
                     for (int u = 0; u < VPT; ++u) {
                         const uint32_t vi = base + threadIdx.x + u * blockDim.x;
                         if (vi < nvec) {
-                            const u4* q0 = reinterpret_cast<const u4*>(s0 + (uint64_t)vi * 16);
-                            const u4* q1 = reinterpret_cast<const u4*>(s1 + (uint64_t)vi * 16);
+                            const BINE_GLOBAL u4* q0 = gptr<const u4>(s0 + (uint64_t)vi * 16);
+                            const BINE_GLOBAL u4* q1 = gptr<const u4>(s1 + (uint64_t)vi * 16);
                             a[u] = NT ? __builtin_nontemporal_load(q0) : *q0;
                             c[u] = NT ? __builtin_nontemporal_load(q1) : *q1;
                         }
@@ -279,8 +324,8 @@ __global__ void __launch_bounds__(512) grun_kernel(const GredItem* __restrict__ 
                     for (int u = 0; u < VPT; ++u) {
                         const uint32_t vi = base + threadIdx.x + u * blockDim.x;
                         if (vi < nvec) {
-                            u4* q0 = reinterpret_cast<u4*>(d0 + (uint64_t)vi * 16);
-                            u4* q1 = reinterpret_cast<u4*>(d1 + (uint64_t)vi * 16);
+                            BINE_GLOBAL u4* q0 = gptr<u4>(d0 + (uint64_t)vi * 16);
+                            BINE_GLOBAL u4* q1 = gptr<u4>(d1 + (uint64_t)vi * 16);
                             if (NT) {
                                 __builtin_nontemporal_store(a[u], q0);
                                 __builtin_nontemporal_store(c[u], q1);
@@ -294,9 +339,9 @@ __global__ void __launch_bounds__(512) grun_kernel(const GredItem* __restrict__ 
             } else {
                 const uint64_t nb = (uint64_t)it.n * esize;
                 for (uint64_t b = threadIdx.x; b < nb; b += blockDim.x) {
-                    const char x0 = s0[b], x1 = s1[b];
-                    d0[b] = x0;
-                    d1[b] = x1;
+                    const char x0 = *gptr<const char>(s0 + b), x1 = *gptr<const char>(s1 + b);
+                    *gptr<char>(d0 + b) = x0;
+                    *gptr<char>(d1 + b) = x1;
                 }
             }
         }
@@ -318,18 +363,20 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
     Vec<T>* lds = reinterpret_cast<Vec<T>*>(lds_raw);
     const uint32_t nw = blockDim.x >> 6;
     const uint32_t lgw = __builtin_ctz(nw);
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    // the wave index is the same on every lane of a wave: said so, the output loop `o = wave; o += nw` reads
+    // its pool entries with scalar loads
+    const uint32_t lane = threadIdx.x & 63u, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (uint32_t ii = blockIdx.x; ii < nitems; ii += gridDim.x) {
-        const GredItem it = items[ii];
+        const GredItem it = cload(cptr(items + ii));
         Table t;
         if (BATCH) {
-            const BufTable& bt = tabs[it.bucket];
-            t = Table{bt.base[0], bt.base[1], bt.base[2], bt.base[3], bt.stride[0], bt.stride[1], bt.stride[2], bt.stride[3]};
+            const BINE_CONST BufTable* bt = cptr(tabs + it.bucket);
+            t = Table{bt->base[0], bt->base[1], bt->base[2], bt->base[3], bt->stride[0], bt->stride[1], bt->stride[2], bt->stride[3]};
         } else {
             t = Table{tab.base[0], tab.base[1], tab.base[2], tab.base[3], tab.stride[0], tab.stride[1], tab.stride[2], tab.stride[3]};
         }
-        const PoolEntry* __restrict__ leaves = pool + it.leaf_off;
-        const PoolEntry* __restrict__ outs = pool + it.out_off;
+        const BINE_CONST PoolEntry* leaves = cptr(pool + it.leaf_off);
+        const BINE_CONST PoolEntry* outs = cptr(pool + it.out_off);
         const uint32_t lg = it.lg;
         const uint64_t delta = it.delta;
         auto addp = [](const Pack<T, VPT>& a, const Pack<T, VPT>& b) __attribute__((always_inline)) {
@@ -344,7 +391,7 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
             // a whole subtree of the balanced tree; the top lgw levels are then
             // joined pairwise from LDS, so the association is unchanged.
             const uint32_t lgs = lg - lgw;
-            const PoolEntry* __restrict__ my = leaves + ((size_t)wave << lgs);
+            const BINE_CONST PoolEntry* my = cptr(pool + it.leaf_off + ((size_t)wave << lgs));
             const uint32_t nvec = it.n / VE;
             for (uint32_t base = 0; base < nvec; base += 64 * VPT) {
                 uint32_t vi[VPT];
@@ -355,7 +402,7 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
                     ok[u] = vi[u] < nvec;
                 }
                 auto loadp = [&](uint32_t k) __attribute__((always_inline)) {
-                    const PoolEntry e = my[k];
+                    const PoolEntry e = cload(my + k);
                     const char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
                     Pack<T, VPT> r;
 #pragma unroll
@@ -393,7 +440,7 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
                         rot[u] = rotate_prev(v[u], lane);
                     }
                     for (uint32_t o = wave; o < it.nouts; o += nw) {
-                        const PoolEntry e = outs[o];
+                        const PoolEntry e = cload(outs + o);
                         char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
                         const uint32_t m = (uint32_t)(uintptr_t)p & 15u;
                         if (m != 0 && (m & 3u) == 0) {
@@ -406,7 +453,7 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
                     }
                 } else {
                     for (uint32_t o = wave; o < it.nouts; o += nw) {
-                        const PoolEntry e = outs[o];
+                        const PoolEntry e = cload(outs + o);
                         char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
 #pragma unroll
                         for (int u = 0; u < VPT; ++u)
@@ -426,7 +473,7 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
                     ok[u] = vi[u] < nvec;
                 }
                 auto loadp = [&](uint32_t k) __attribute__((always_inline)) {
-                    const PoolEntry e = leaves[k];
+                    const PoolEntry e = cload(leaves + k);
                     const char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
                     Pack<T, VPT> r;
 #pragma unroll
@@ -446,7 +493,7 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
                         rot[u] = rotate_prev(v[u], lane);
                     }
                     for (uint32_t o = 0; o < it.nouts; ++o) {
-                        const PoolEntry e = outs[o];
+                        const PoolEntry e = cload(outs + o);
                         char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
                         const uint32_t m = (uint32_t)(uintptr_t)p & 15u;
                         if (m != 0 && (m & 3u) == 0) {
@@ -459,7 +506,7 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
                     }
                 } else {
                     for (uint32_t o = 0; o < it.nouts; ++o) {
-                        const PoolEntry e = outs[o];
+                        const PoolEntry e = cload(outs + o);
                         char* p = t.at(e.rb, (uint64_t)e.off + delta, sizeof(T));
 #pragma unroll
                         for (int u = 0; u < VPT; ++u)
@@ -471,13 +518,13 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
             for (uint32_t e0 = threadIdx.x; e0 < it.n; e0 += blockDim.x) {
                 auto adds = [](T a, T b) __attribute__((always_inline)) { return add1<T>(a, b); };
                 auto loads = [&](uint32_t k) __attribute__((always_inline)) {
-                    const PoolEntry e = leaves[k];
-                    return *reinterpret_cast<const T*>(t.at(e.rb, (uint64_t)e.off + delta + e0, sizeof(T)));
+                    const PoolEntry e = cload(leaves + k);
+                    return *gptr<const T>(t.at(e.rb, (uint64_t)e.off + delta + e0, sizeof(T)));
                 };
                 T s = tree_eval<T, MAXS, 8>(lg, adds, loads);
                 for (uint32_t o = 0; o < it.nouts; ++o) {
-                    const PoolEntry e = outs[o];
-                    *reinterpret_cast<T*>(t.at(e.rb, (uint64_t)e.off + delta + e0, sizeof(T))) = s;
+                    const PoolEntry e = cload(outs + o);
+                    *gptr<T>(t.at(e.rb, (uint64_t)e.off + delta + e0, sizeof(T))) = s;
                 }
             }
         }
@@ -516,7 +563,7 @@ __global__ void __launch_bounds__(256) reduce_local_kernel(const T* a, const T* 
         }
         done = nvec * (uint64_t)Vec<T>::N;
     }
-    for (uint64_t i = done + tid; i < n; i += lanes) out[i] = add1<T>(a[i], b[i]);
+    for (uint64_t i = done + tid; i < n; i += lanes) *gptr<T>(out + i) = add1<T>(*gptr<const T>(a + i), *gptr<const T>(b + i));
 }
 
 // ---- host-side dispatch ------------------------------------------------------
@@ -550,6 +597,8 @@ static hipError_t launch_v(int max_lg, const GredItem* items, uint32_t n, const 
         return small ? launch_t<T, 2, 6, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st)
                      : launch_t<T, 2, 11, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st);
     default:  // four vectors per thread: groups of 4 leaves, one more stack level; deep trees run with two
+        // trees of up to 64 leaves need four stack levels: 163 VGPRs, three waves per SIMD instead of two
+        if (max_lg <= 6) return launch_t<T, 4, 4, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st);
         return small ? launch_t<T, 4, 7, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st)
                      : launch_t<T, 2, 11, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st);
     }

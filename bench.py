@@ -76,6 +76,21 @@ def time_launches(fn, stream, steps, warmup):
     return wall * 1e3, e0.elapsed_time(e1) / steps
 
 
+def launch_spread(fn, stream, n):
+    """Milliseconds of n launches timed one by one (HIP events around each, synchronised in between)."""
+    import torch
+    ts = []
+    for _ in range(n):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return ts
+
+
 def cpu_baseline(warmup, runs):
     """The oracle (C restatement of the reference's MPI-layer allreduce B over P
     virtual ranks; message copies and reductions spread over OpenMP threads) on the
@@ -410,9 +425,11 @@ def per_collective_8x8x8(stream):
                 fn = lambda: comm.reduce(send, recv, stream=stream)  # noqa: E731
             comm.autotune(name, send, recv, count, exec="FUSED", iters=5, stream=stream)
             w, ev = time_launches(fn, stream, 10, 2)
+            one = launch_spread(fn, stream, 5)
             out[name] = {"algorithmic_bytes": int(b_alg), "ms": round(ev, 4),
                          "GBps": round(b_alg / (ev * 1e-3) / 1e9, 1),
-                         "pct_hbm_peak": round(100 * b_alg / (ev * 1e-3) / 1e9 / HBM_PEAK_GBPS, 1)}
+                         "pct_hbm_peak": round(100 * b_alg / (ev * 1e-3) / 1e9 / HBM_PEAK_GBPS, 1),
+                         "single_launch_ms_min_max": [round(min(one), 4), round(max(one), 4)]}
             del send, recv
         comm.close()
     torch.cuda.synchronize()

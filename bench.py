@@ -471,7 +471,7 @@ def linksim_tori(stream):
             "link_crossings_per_pass": crossings, "ms_per_pass": round(ms, 4),
             "gpu_cases_per_s": round(len(cases) / (ms * 1e-3), 1), "gpu_classes_per_s": round(classes / (ms * 1e-3), 1),
             "samples_per_s": round(samples / (ms * 1e-3), 1), "sizes_per_case": 26,
-            "atomic_adds_per_s": round(crossings / (ms * 1e-3), 1)}
+            "link_crossings_per_s": round(crossings / (ms * 1e-3), 1)}
 
 
 def selector_regret():

@@ -14,7 +14,8 @@
 
 namespace bine {
 
-hipError_t launch_sim(const void* cases, const void* classes, uint32_t ncls, uint32_t max_p, uint32_t* mult,
+hipError_t launch_sim(const void* cases, const void* classes, uint32_t ncls, const void* count_blocks,
+                      uint32_t n_count_blocks, const void* scan_blocks, uint32_t n_scan_blocks, uint32_t* mult,
                       uint64_t mult_len, uint32_t* max_hops, uint32_t* max_mult, uint64_t* sum_mult, hipStream_t st);
 
 static simd::TorusD dev_torus(const Torus& t) {
@@ -61,6 +62,9 @@ bool sim_build_case(const Torus& t, int coll, int algo, SimCase* out) {
         if (algo == SIM_SIMPY && coll != BINE_COLL_ALLGATHER) {
             for (uint32_t i = 1; i < t.nd; ++i)
                 if (t.dims[i] != t.dims[0]) return false;  // simulator.py cycles dimensions step % D
+            // step s takes distances[s // D] from a table of 10 (simulator.py:195), so at most 10 steps per
+            // dimension; simulator.py raises IndexError beyond, and so does the case here
+            if (t.steps_per_dim[0] > 10) return false;
             for (uint32_t s = 0; s < L; ++s) {
                 if (coll == BINE_COLL_ALLREDUCE) add(SK_SIMPY, 0, s, 1, 1, 1);  // full vector (simulator.py:109-112)
                 else add(SK_SIMPY, 0, s, 1, 2u << s, 1);                        // find_reducescatter_indexes
@@ -121,14 +125,18 @@ struct bine_sim_s {
     SimCaseRec* d_cases = nullptr;
     SimClassRec* d_classes = nullptr;
     uint32_t* d_mult = nullptr;
+    uint32_t* d_count_blocks = nullptr;  // per workgroup of the count kernel: (class, first rank)
+    uint32_t* d_scan_blocks = nullptr;   // per workgroup of the ring scan: (class, first ring-direction)
+    uint32_t n_count_blocks = 0, n_scan_blocks = 0;
     uint32_t* d_hops = nullptr;
     uint32_t* d_max = nullptr;
     uint64_t* d_sum = nullptr;
     uint64_t mult_len = 0;
-    uint32_t ncls = 0, max_p = 0;
+    uint32_t ncls = 0;
     hipStream_t last = nullptr;  // stream of the last run; queries wait for it
     ~bine_sim_s() {
-        for (void* p : {(void*)d_cases, (void*)d_classes, (void*)d_mult, (void*)d_hops, (void*)d_max, (void*)d_sum})
+        for (void* p : {(void*)d_cases, (void*)d_classes, (void*)d_mult, (void*)d_hops, (void*)d_max, (void*)d_sum,
+                        (void*)d_count_blocks, (void*)d_scan_blocks})
             if (p) (void)hipFree(p);
     }
 };
@@ -176,6 +184,7 @@ extern "C" int bine_sim_create(const bine_sim_case_t* cases, int n, int device, 
     s->device = device;
     std::vector<SimCaseRec> crec;
     std::vector<SimClassRec> krec;
+    std::vector<uint32_t> cblk, sblk;  // (class, first index) pairs
     uint64_t off = 0;
     for (int i = 0; i < n; ++i) {
         Torus t;
@@ -187,12 +196,22 @@ extern "C" int bine_sim_create(const bine_sim_case_t* cases, int n, int device, 
         s->class_first.push_back((uint32_t)krec.size());
         s->nlinks.push_back(nl);
         crec.push_back(SimCaseRec{{{t.dims[0], t.dims[1], t.dims[2]}, t.nd, t.size}, (uint32_t)ceil_log2(t.size), nl, 0});
+        uint32_t rings = 0;  // ring-directions of the torus: 2 * size / dims[d] per dimension
+        for (uint32_t d = 0; d < t.nd; ++d) rings += 2 * (t.size / t.dims[d]);
         for (const SimClass& c : sc.classes) {
+            const uint32_t cls = (uint32_t)krec.size();
             krec.push_back(SimClassRec{c.kind, c.d, c.k, (uint32_t)i, off});
             s->mult_off.push_back(off);
             off += nl;
+            for (uint32_t r = 0; r < t.size; r += 256) {
+                cblk.push_back(cls);
+                cblk.push_back(r);
+            }
+            for (uint32_t q = 0; q < rings; q += 256) {
+                sblk.push_back(cls);
+                sblk.push_back(q);
+            }
         }
-        s->max_p = std::max(s->max_p, t.size);
         s->cases.push_back(std::move(sc));
     }
     s->ncls = (uint32_t)krec.size();
@@ -203,6 +222,12 @@ extern "C" int bine_sim_create(const bine_sim_case_t* cases, int n, int device, 
     SHIP(hipMalloc(&s->d_classes, sizeof(SimClassRec) * krec.size()));
     SHIP(hipMemcpy(s->d_classes, krec.data(), sizeof(SimClassRec) * krec.size(), hipMemcpyHostToDevice));
     SHIP(hipMalloc(&s->d_mult, sizeof(uint32_t) * off));
+    s->n_count_blocks = (uint32_t)(cblk.size() / 2);
+    s->n_scan_blocks = (uint32_t)(sblk.size() / 2);
+    SHIP(hipMalloc(&s->d_count_blocks, sizeof(uint32_t) * cblk.size()));
+    SHIP(hipMemcpy(s->d_count_blocks, cblk.data(), sizeof(uint32_t) * cblk.size(), hipMemcpyHostToDevice));
+    SHIP(hipMalloc(&s->d_scan_blocks, sizeof(uint32_t) * sblk.size()));
+    SHIP(hipMemcpy(s->d_scan_blocks, sblk.data(), sizeof(uint32_t) * sblk.size(), hipMemcpyHostToDevice));
     SHIP(hipMalloc(&s->d_hops, sizeof(uint32_t) * s->ncls));
     SHIP(hipMalloc(&s->d_max, sizeof(uint32_t) * s->ncls));
     SHIP(hipMalloc(&s->d_sum, sizeof(uint64_t) * s->ncls));
@@ -213,8 +238,8 @@ extern "C" int bine_sim_create(const bine_sim_case_t* cases, int n, int device, 
 extern "C" int bine_sim_run(bine_sim_t s, void* stream) {
     if (!s) return sfail(BINE_ERR_ARG, "NULL simulation");
     SHIP(hipSetDevice(s->device));
-    SHIP(launch_sim(s->d_cases, s->d_classes, s->ncls, s->max_p, s->d_mult, s->mult_len, s->d_hops, s->d_max, s->d_sum,
-                    (hipStream_t)stream));
+    SHIP(launch_sim(s->d_cases, s->d_classes, s->ncls, s->d_count_blocks, s->n_count_blocks, s->d_scan_blocks,
+                    s->n_scan_blocks, s->d_mult, s->mult_len, s->d_hops, s->d_max, s->d_sum, (hipStream_t)stream));
     s->last = (hipStream_t)stream;
     return BINE_OK;
 }

@@ -134,9 +134,10 @@ BINE_HD inline uint32_t dest(const TorusD& t, uint32_t kind, uint32_t d, uint32_
         int c[3];
         simpy_coords(t, r, c);
         const uint32_t dim = k % t.nd, rel = k / t.nd;
+        if (rel >= 10) return 0xFFFFFFFFu;  // beyond simulator.py's table; such cases are refused when built
         int sgn = (rel % 2 == 0) ? 1 : -1;
         if (c[dim] % 2) sgn = -sgn;
-        c[dim] = pm(c[dim] + sgn * dist_tab[rel < 10 ? rel : 9], (int)t.dims[dim]);
+        c[dim] = pm(c[dim] + sgn * dist_tab[rel], (int)t.dims[dim]);
         return simpy_ident(t, c);
     }
     default: return 0xFFFFFFFFu;

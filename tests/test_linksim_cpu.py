@@ -94,6 +94,17 @@ def test_undefined_cases_rejected():
             ls.classes(dims, coll, algo)
 
 
+def test_simpy_beyond_its_distance_table_rejected():
+    """simulator.py takes step s's distance from a 10-entry table (simulator.py:195): 10 steps per dimension
+    (a ring of 1024, or 32x32) work, 11 (a ring of 2048) raise IndexError there and are refused here."""
+    from bine_trees_fugaku_amd._lib import BineError
+    assert len(ls.classes([1024], "allreduce", "simpy")["kind"]) == 10
+    assert len(ls.classes([32, 32], "reduce_scatter", "simpy")["kind"]) == 10
+    for dims in ([2048], [2048, 2048]):
+        with pytest.raises(BineError):
+            ls.classes(dims, "allreduce", "simpy")
+
+
 def test_class_shares():
     """Closed forms: reduce-scatter halves (sum (P-1)/P), allgather doubles, ring repeats P-1 per phase."""
     from fractions import Fraction as F

@@ -331,6 +331,14 @@ def test_sim_run_results_and_stats(dims):
     assert stats[1] == sum(2 ** (s / D) for s in range(L))
     assert stats[2] == sum(d / 2 ** (s + 1) for s, d in enumerate(dist))
     assert stats[4] == 1 - 2.0 ** -L
+    # bw_ref (simulator.py:168,172) = sum_s 2^-(s+1) 2^(s/D): a geometric series with ratio q = 2^(1/D - 1),
+    # (1 - q^L) / (2 (1 - q)); for D = 1 every term is 1/2
+    q = 2.0 ** (1.0 / D - 1.0)
+    closed = L / 2 if D == 1 else (1 - q ** L) / (2 * (1 - q))
+    assert abs(stats[3] - closed) <= 1e-14 * closed
+    if D == 1:
+        assert stats[3] == L / 2
+
 
 
 def test_remap_rank_against_golden_fixture():

@@ -1189,9 +1189,24 @@ bool build_program(const Torus& t, const bine_vcomm_config_t& cfg, const CollReq
         return false;
     }
     auto fam_ok = [&](int f) { return f == FAM_BINE || f == FAM_RECDOUB; };
+    // Family DEFAULT: the reference hands the call to the vendor MPI (PMPI_*, lib/libbine.cc:1016, 1069,
+    // 1174, 1257, 1295, 1317, 1381, 1445).  The stand-in here is the recursive-doubling family for the
+    // torus collectives on power-of-two tori (recursive halving then doubling: the classic vendor
+    // allreduce), Bruck for the alltoall, and the Bine trees for the rooted collectives, whose MPI-layer
+    // code the reference runs for BINE and RECDOUB alike.
+    bool all_p2 = true;
+    for (uint32_t d = 0; d < t.nd; ++d) all_p2 = all_p2 && is_pow2(t.dims[d]);
+    auto resolve = [&](bine_coll_config_t c, int coll) {
+        if (c.family != FAM_DEFAULT) return c;
+        if (coll == BINE_COLL_ALLTOALL) c.family = FAM_BRUCK;
+        else if (coll == BINE_COLL_ALLREDUCE || coll == BINE_COLL_REDUCE_SCATTER || coll == BINE_COLL_ALLGATHER)
+            c.family = all_p2 ? FAM_RECDOUB : FAM_BINE;
+        else c.family = FAM_BINE;
+        return c;
+    };
     switch (req.coll) {
     case BINE_COLL_ALLREDUCE: {
-        const bine_coll_config_t& c = cfg.allreduce;
+        const bine_coll_config_t c = resolve(cfg.allreduce, req.coll);
         if (!fam_ok(c.family)) { *err = "allreduce family must be BINE or RECDOUB"; return false; }
         bool ok;
         switch (c.algo) {
@@ -1214,14 +1229,14 @@ bool build_program(const Torus& t, const bine_vcomm_config_t& cfg, const CollReq
         return ok;
     }
     case BINE_COLL_REDUCE_SCATTER: {
-        const bine_coll_config_t& c = cfg.reduce_scatter;
+        const bine_coll_config_t c = resolve(cfg.reduce_scatter, req.coll);
         if (!fam_ok(c.family)) { *err = "reduce_scatter family must be BINE or RECDOUB"; return false; }
         if (c.algo != 0) { *err = "reduce_scatter algorithm must be VEC_HALVING_CONT_PERMUTE"; return false; }
         if (req.count * (int64_t)P < (int64_t)ports * P) { *err = "count < num_ports * P"; return false; }
         return reduce_scatter_cont(t, ports, c.family, c.distance, req.count, req.op, p, err);
     }
     case BINE_COLL_ALLGATHER: {
-        const bine_coll_config_t& c = cfg.allgather;
+        const bine_coll_config_t c = resolve(cfg.allgather, req.coll);
         if (!fam_ok(c.family)) { *err = "allgather family must be BINE or RECDOUB"; return false; }
         if (req.count < 1) { *err = "allgather needs count >= 1"; return false; }
         if (c.algo == BINE_ALLGATHER_CONT_PERMUTE) {
@@ -1237,7 +1252,7 @@ bool build_program(const Torus& t, const bine_vcomm_config_t& cfg, const CollReq
         return false;
     }
     case BINE_COLL_ALLTOALL: {
-        const bine_coll_config_t& c = cfg.alltoall;
+        const bine_coll_config_t c = resolve(cfg.alltoall, req.coll);
         if (req.count < 1) { *err = "alltoall needs count >= 1"; return false; }
         if (c.family == FAM_BRUCK) return alltoall_bruck(P, req.count, p, err);
         if (c.family == FAM_BINE) return alltoall_bine(P, req.count, p, err);
@@ -1245,7 +1260,7 @@ bool build_program(const Torus& t, const bine_vcomm_config_t& cfg, const CollReq
         return false;
     }
     case BINE_COLL_BCAST: {
-        const bine_coll_config_t& c = cfg.bcast;
+        const bine_coll_config_t c = resolve(cfg.bcast, req.coll);
         if (!fam_ok(c.family)) { *err = "bcast family must be BINE or RECDOUB"; return false; }
         if (req.root < 0 || req.root >= (int)P) { *err = "root out of range"; return false; }
         if (c.algo == BINE_BCAST_BINOMIAL_TREE) return bcast_tree(P, req.count, req.root, p, err);
@@ -1257,7 +1272,7 @@ bool build_program(const Torus& t, const bine_vcomm_config_t& cfg, const CollReq
         return false;
     }
     case BINE_COLL_REDUCE: {
-        const bine_coll_config_t& c = cfg.reduce;
+        const bine_coll_config_t c = resolve(cfg.reduce, req.coll);
         if (!fam_ok(c.family)) { *err = "reduce family must be BINE or RECDOUB"; return false; }
         if (req.root < 0 || req.root >= (int)P) { *err = "root out of range"; return false; }
         if (c.algo == BINE_REDUCE_BINOMIAL_TREE) return reduce_tree(P, req.count, req.root, req.op, p, err);
@@ -1266,14 +1281,14 @@ bool build_program(const Torus& t, const bine_vcomm_config_t& cfg, const CollReq
         return false;
     }
     case BINE_COLL_SCATTER: {
-        const bine_coll_config_t& c = cfg.scatter;
+        const bine_coll_config_t c = resolve(cfg.scatter, req.coll);
         if (!fam_ok(c.family)) { *err = "scatter family must be BINE or RECDOUB"; return false; }
         if (req.root < 0 || req.root >= (int)P) { *err = "root out of range"; return false; }
         if (req.count < ports) { *err = "recvcount < num_ports (lib/libbine.cc:1329)"; return false; }
         return scatter_tree(P, req.count, req.root, p, err);
     }
     case BINE_COLL_GATHER: {
-        const bine_coll_config_t& c = cfg.gather;
+        const bine_coll_config_t c = resolve(cfg.gather, req.coll);
         if (!fam_ok(c.family)) { *err = "gather family must be BINE or RECDOUB"; return false; }
         if (req.root < 0 || req.root >= (int)P) { *err = "root out of range"; return false; }
         if (req.count < ports) { *err = "recvcount < num_ports (lib/libbine.cc:1393)"; return false; }

@@ -271,3 +271,18 @@ def test_errors_do_not_launch(gpu):
         with pytest.raises(BineError):
             c.gather(s[:, :4].contiguous(), torch.zeros(64, device="cuda"), root=1)  # odd root unsupported
     torch.cuda.synchronize()
+
+
+def test_family_default_stand_in(gpu):
+    """Family DEFAULT (the reference's PMPI_* fallback, lib/libbine.cc:1016) runs recursive doubling for the
+    allreduce on a power-of-two torus and Bruck for the alltoall: compared with the oracle of those schedules."""
+    x = per_rank(64, 64 * 5 + 3, "float64", 77)
+    with VComm([4, 4, 4], allreduce_family="DEFAULT") as c:
+        r = torch.zeros_like(dev(x))
+        c.allreduce(dev(x), r)
+        assert same_bits(host(r), O.allreduce(x, [4, 4, 4], family="RECDOUB"))
+    a = per_rank(16, 16 * 7, "int64", 78)
+    with VComm([16], alltoall_family="DEFAULT") as c:
+        r = torch.zeros_like(dev(a))
+        c.alltoall(dev(a), r)
+        assert same_bits(host(r), O.alltoall(a, 16, family="BRUCK"))

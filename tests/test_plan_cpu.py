@@ -335,3 +335,23 @@ def test_allreduce_in_place_needs_its_own_schedule():
 def test_in_place_rejected_for_other_collectives():
     with pytest.raises(BineError):
         plan_dump([4, 4], "reduce_scatter", 4, "int32", in_place=True)
+
+
+# ---- family DEFAULT (the reference's vendor-MPI fallback) ---------------------------------
+
+DEFAULT_STANDIN = [("allreduce", [4, 4, 4], "RECDOUB"), ("allreduce", [6, 4], "BINE"), ("reduce_scatter", [4, 4], "RECDOUB"),
+                   ("allgather", [2, 2, 2], "RECDOUB"), ("alltoall", [16], "BRUCK"), ("bcast", [8], "BINE"),
+                   ("reduce", [16], "BINE"), ("scatter", [8], "BINE"), ("gather", [8], "BINE")]
+
+
+@pytest.mark.parametrize("coll,dims,standin", DEFAULT_STANDIN)
+def test_family_default_runs_its_stand_in(coll, dims, standin):
+    """LIBBINE_<COLL>_ALGO_FAMILY=DEFAULT calls PMPI_* in the reference (lib/libbine.cc:1016): here it compiles
+    the stand-in schedule, work item for work item."""
+    P = int(np.prod(dims))
+    count = 4 * P + (3 if coll in ("allreduce", "bcast", "reduce") else 0)
+    a = plan_dump(dims, coll, count, "float32", exec="STEP", **{f"{coll}_family": "DEFAULT"})
+    b = plan_dump(dims, coll, count, "float32", exec="STEP", **{f"{coll}_family": standin})
+    assert a[0] == b[0]
+    for x, y in zip(a[1:], b[1:]):
+        assert x.tobytes() == y.tobytes()

@@ -143,15 +143,61 @@ class Run:
                 samples.append(time.perf_counter() - t0)
         return samples
 
+    def expected(self):
+        """The collective's MPI result computed directly with torch on the same device buffers: an
+        implementation independent of the library, as the reference validates against the system MPI
+        (bench/bench.cc:270, 328-344).  Integer sums are formed in int64 and wrapped to the element type;
+        float sums in float64."""
+        torch = self.torch
+        P = self.comm.size
+        k = self.collective
+        x = self.send
+        wide = torch.float64 if x.dtype.is_floating_point else torch.int64
+
+        def narrow(v):
+            if x.dtype.is_floating_point:
+                return v
+            bits = torch.iinfo(x.dtype).bits
+            v = torch.remainder(v + (1 << (bits - 1)), 1 << bits) - (1 << (bits - 1))
+            return v.to(x.dtype)
+
+        if k == "MPI_Allreduce":
+            return narrow(x.to(wide).sum(0)).expand(P, -1)
+        if k == "MPI_Reduce":
+            return narrow(x.to(wide).sum(0))
+        if k == "MPI_Reduce_scatter":
+            return narrow(x.to(wide).sum(0)).reshape(P, -1)
+        if k == "MPI_Allgather":
+            return x.reshape(1, -1).expand(P, -1)
+        if k == "MPI_Gather":
+            return x.reshape(-1)
+        if k == "MPI_Bcast":
+            return x[0].expand(P, -1)
+        if k == "MPI_Scatter":
+            return x.reshape(P, -1)
+        c = x.shape[1] // P  # MPI_Alltoall: block j of rank i lands as block i of rank j
+        return x.reshape(P, P, c).transpose(0, 1).reshape(P, P * c)
+
     def validate(self) -> bool:
-        """The reference validates against the system MPI (PMPI_*).  Here the
-        timed output is compared bit for bit with the same collective run one
-        launch per reference step on fresh buffers (skipped for the VOID op,
-        as bench/bench.cc:334 does)."""
+        """Compare the timed output with the directly computed MPI result (exact for integer types; for
+        floats within the rounding of P - 1 additions, |err| <= (P - 1) * eps * sum |x|), and bit for bit
+        with the same collective run one launch per reference step on fresh buffers.  Skipped for the VOID
+        op, as bench/bench.cc:334 does."""
         if self.op == "VOID":
             return True
         torch = self.torch
         got = self.recv.clone()
+        want = self.expected()
+        if got.dtype.is_floating_point and self.collective in ("MPI_Allreduce", "MPI_Reduce", "MPI_Reduce_scatter"):
+            mag = self.send.to(torch.float64).abs().sum(0)
+            if self.collective == "MPI_Reduce_scatter":
+                mag = mag.reshape(self.comm.size, -1)
+            eps = torch.finfo(got.dtype).eps
+            ok = bool(((got.to(torch.float64) - want).abs() <= (self.comm.size - 1) * eps * mag).all())
+        else:
+            ok = torch.equal(got.reshape(want.shape).view(torch.uint8), want.contiguous().view(torch.uint8))
+        if not ok:
+            return False
         saved = self.recv
         self.recv = torch.zeros_like(saved) if self.collective != "MPI_Bcast" else self.send.clone()
         with self.comm.with_exec("STEP") as step:

@@ -255,9 +255,11 @@ def test_batch_rejects_bad_items(gpu):
 @pytest.mark.parametrize("collective,type_name,count", [
     ("MPI_Allreduce", "INT32", 262144), ("MPI_Reduce_scatter", "FLOAT", 4096), ("MPI_Allgather", "INT", 1024),
     ("MPI_Bcast", "CHAR", 65536), ("MPI_Alltoall", "INT32", 64), ("MPI_Scatter", "FLOAT", 1024),
-    ("MPI_Gather", "INT32", 1024), ("MPI_Reduce", "VOID", 65536), ("MPI_Allreduce", "DOUBLE", 4099)])
+    ("MPI_Gather", "INT32", 1024), ("MPI_Reduce", "VOID", 65536), ("MPI_Allreduce", "DOUBLE", 4099),
+    ("MPI_Reduce", "INT32", 1000), ("MPI_Allreduce", "CHAR", 4096), ("MPI_Allreduce", "FLOAT", 5000)])
 def test_harness_runs_and_validates(gpu, collective, type_name, count, capsys):
-    """bench/bench.cc's loop on the GPU: record printed, timed output equal to per-step execution."""
+    """bench/bench.cc's loop on the GPU: record printed, timed output equal to the MPI result computed directly
+    with torch (exact for integer types, char sums wrapping) and to per-step execution."""
     from bine_trees_fugaku_amd import harness as H
     with VComm([2, 2, 2]) as comm:
         run = H.Run(collective, type_name, count, comm=comm)

@@ -358,6 +358,15 @@ def small_configs(stream, cpu=True):
             res["graph_replay_ms"] = round(w, 4)
             bt.close()
             c.close()
+            # the per-step schedule (one launch per step) issued directly and as one captured graph
+            c = VComm(dims, exec="STEP")
+            bt = Batch(c, [{"coll": "allreduce", "sendbuf": x, "recvbuf": y, "count": count}])
+            w, _ = time_launches(lambda: bt.launch(stream), stream, 20, 3)
+            res["step_direct_ms"] = round(w, 4)
+            w, _ = time_launches(lambda: bt.launch_graph(stream), stream, 20, 3)
+            res["step_graph_ms"] = round(w, 4)
+            bt.close()
+            c.close()
         torch.cuda.synchronize()
         if cpu:
             from oracle import oracle as O

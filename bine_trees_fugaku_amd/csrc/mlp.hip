@@ -7,7 +7,9 @@
 //
 // Every product is one tiled GEMM on the matrix cores: a 64x64 output tile
 // per 256-thread workgroup, each wave owning a 32x32 quarter built from 2x2
-// v_mfma_f32_16x16x4f32 blocks, operands staged through LDS in K chunks of 16.
+// v_mfma_f32_16x16x4f32 blocks, operands staged through LDS in K chunks of 64
+// (16-byte loads and LDS stores), the next chunk loaded into registers while
+// the current one is multiplied.
 // The loss is a masked mean squared error; every reduction runs in a fixed
 // order, so gradients are deterministic run to run.
 #include <hip/hip_runtime.h>
@@ -28,7 +30,7 @@ namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int TM = 64, TN = 64, TK = 16;
+constexpr int TM = 64, TN = 64, TK = 64;
 
 enum Epi : int {
     EPI_BIAS = 0,       // C = acc + bias[col]
@@ -37,91 +39,212 @@ enum Epi : int {
     EPI_PLAIN = 3,      // C = acc
 };
 
+constexpr int SK = TK + 4;   // LDS row stride of a panel held [row][k]
+constexpr int SR = TM + 16;  // LDS row stride of a panel held [k][row]
+
+// One operand's 64 x 64 panel of op(X): rows [r0, r0 + 64), k in [k0, k0 + 64).
+// Element (r, k) is X[r * ld + k] when KC (k contiguous in memory), else
+// X[k * ld + r].  Thread t holds four quads: quad q has the outer index
+// t / 16 + 16 q and the inner indices 4 (t % 16) + 0..3, the inner index
+// running along X's contiguous dimension, so a whole panel moves as 16-byte
+// loads and 16-byte LDS stores.  LDS holds the panel in X's own orientation:
+// [r][k] with row stride SK when KC, [k][r] with row stride SR otherwise; both
+// strides make the fragment reads of the products free of bank conflicts.
+template <bool KC>
+struct Panel {
+    const float* X;
+    int ld, r0, R;
+    bool vec;  // X and ld allow 16-byte quads
+
+    __device__ __forceinline__ size_t at(int r, int k) const { return KC ? (size_t)r * ld + k : (size_t)k * ld + r; }
+    __device__ __forceinline__ void rk(int k0, int o, int c, int& r, int& k) const {
+        if (KC) {
+            r = r0 + o;
+            k = k0 + c;
+        } else {
+            k = k0 + o;
+            r = r0 + c;
+        }
+    }
+    // the whole panel lies inside the operand and moves as quads (the same for the whole workgroup)
+    __device__ __forceinline__ bool whole(int k0, int kend) const { return vec && r0 + TM <= R && k0 + TK <= kend; }
+
+    __device__ __forceinline__ void fetch(floatx4 (&v)[4], int t, int k0, int kend, int kbeg) const {
+        if (whole(k0, kend)) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                int r, k;
+                rk(k0, t / 16 + 16 * q, 4 * (t % 16), r, k);
+                v[q] = *reinterpret_cast<const floatx4*>(X + at(r, k));
+            }
+        } else {  // element by element; an element out of range loads the panel's first one, replaced when staged
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    int r, k;
+                    rk(k0, t / 16 + 16 * q, 4 * (t % 16) + c, r, k);
+                    const bool ok = r < R && k < kend;
+                    v[q][c] = X[ok ? at(r, k) : at(r0, kbeg)];
+                }
+        }
+    }
+    __device__ __forceinline__ void stage(float* S, const floatx4 (&v)[4], int t, int k0, int kend) const {
+        const bool w = whole(k0, kend);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            floatx4 x = v[q];
+            if (!w) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    int r, k;
+                    rk(k0, t / 16 + 16 * q, 4 * (t % 16) + c, r, k);
+                    if (!(r < R && k < kend)) x[c] = 0.f;
+                }
+            }
+            *reinterpret_cast<floatx4*>(S + (t / 16 + 16 * q) * (KC ? SK : SR) + 4 * (t % 16)) = x;
+        }
+    }
+    // panel element (row rr, k kk), from LDS
+    __device__ __forceinline__ float frag(const float* S, int rr, int kk) const {
+        return KC ? S[rr * SK + kk] : S[kk * SR + rr];
+    }
+};
+
 // C[M x N] = op(A)[M x K] * op(B)[K x N] with the epilogue above.
 // op(A)[m][k] = TA ? A[k * lda + m] : A[m * lda + k]
 // op(B)[k][n] = TB ? B[n * ldb + k] : B[k * ldb + n]
 // Split K: workgroup z covers k in [z * kc, min(K, (z + 1) * kc)) and, when
 // gridDim.z > 1, writes its partial product to C + z * M * ldc (EPI_PLAIN only;
 // sum_partials adds the slices in order).
+//
+// The K loop is software-pipelined: the next chunk's operands are loaded into
+// registers while the matrix cores work on the current chunk from LDS.
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void gemm_f32(int M, int N, int K, int kc, const float* __restrict__ A, int lda,
                                                 const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
                                                 const float* __restrict__ bias, const float* __restrict__ mask, int epi) {
-    __shared__ float As[TK][TM + 4];  // As[k][m]
-    __shared__ float Bs[TK][TN + 4];  // Bs[k][n]
+    __shared__ __align__(16) float As[TK * SR];  // room for either orientation (SR >= SK)
+    __shared__ __align__(16) float Bs[TK * SR];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     const int kbeg = blockIdx.z * kc;
     const int kend = kbeg + kc < K ? kbeg + kc : K;
     C += (size_t)blockIdx.z * M * ldc;
+    const Panel<!TA> pa{A, lda, m0, M, (lda & 3) == 0 && ((uintptr_t)A & 15) == 0};
+    const Panel<TB> pb{B, ldb, n0, N, (ldb & 3) == 0 && ((uintptr_t)B & 15) == 0};
     floatx4 acc[2][2];
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
+    floatx4 va[4], vb[4];
+    pa.fetch(va, tid, kbeg, kend, kbeg);
+    pb.fetch(vb, tid, kbeg, kend, kbeg);
     for (int k0 = kbeg; k0 < kend; k0 += TK) {
-        // 64 x 16 elements of each operand, 4 per thread.
-        for (int e = tid; e < TM * TK; e += 256) {
-            int mm, kk;
-            if (TA) {  // consecutive threads walk m (contiguous in memory)
-                mm = e % TM;
-                kk = e / TM;
-            } else {  // consecutive threads walk k
-                kk = e % TK;
-                mm = e / TK;
-            }
-            const int gm = m0 + mm, gk = k0 + kk;
-            float v = 0.f;
-            if (gm < M && gk < kend) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
-            As[kk][mm] = v;
-        }
-        for (int e = tid; e < TN * TK; e += 256) {
-            int nn, kk;
-            if (TB) {
-                kk = e % TK;
-                nn = e / TK;
-            } else {
-                nn = e % TN;
-                kk = e / TN;
-            }
-            const int gn = n0 + nn, gk = k0 + kk;
-            float v = 0.f;
-            if (gn < N && gk < kend) v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
-            Bs[kk][nn] = v;
-        }
+        pa.stage(As, va, tid, k0, kend);
+        pb.stage(Bs, vb, tid, k0, kend);
         __syncthreads();
+        if (k0 + TK < kend) {  // in flight during the products below
+            pa.fetch(va, tid, k0 + TK, kend, kbeg);
+            pb.fetch(vb, tid, k0 + TK, kend, kbeg);
+        }
+        // 16x16x4: lane supplies A[row = lane % 16][k = lane / 16] and B[k = lane / 16][col = lane % 16].
+        // The whole chunk's fragments are read from LDS first, so the reads wait once, not per product.
+        float af[TK / 4][2], bf[TK / 4][2];
 #pragma unroll
-        for (int ks = 0; ks < TK; ks += 4) {
-            // 16x16x4: lane supplies A[row = lane % 16][k = lane / 16] and B[k = lane / 16][col = lane % 16]
-            const int kk = ks + (lane >> 4);
+        for (int q = 0; q < TK / 4; ++q) {
+            const int kk = 4 * q + (lane >> 4);
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
-                const float a = As[kk][wm + i * 16 + (lane & 15)];
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    const float b = Bs[kk][wn + j * 16 + (lane & 15)];
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i][j], 0, 0, 0);
-                }
+                af[q][i] = pa.frag(As, wm + i * 16 + (lane & 15), kk);
+                bf[q][i] = pb.frag(Bs, wn + i * 16 + (lane & 15), kk);
             }
         }
+#pragma unroll
+        for (int q = 0; q < TK / 4; ++q)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[q][i], bf[q][j], acc[i][j], 0, 0, 0);
         __syncthreads();
     }
     // Output layout of a 16x16 block: lane holds rows 4 * (lane / 16) + r, column lane % 16.
+    // The epilogue's operands (bias, ReLU mask) are loaded unconditionally from clamped, in-range positions
+    // and all values are formed before the first store, so the lane waits for memory once.
+    const bool has_bias = epi == EPI_BIAS || epi == EPI_BIAS_RELU;
+    float bj[2] = {0.f, 0.f};
+    if (has_bias) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = n0 + wn + j * 16 + (lane & 15);
+            bj[j] = bias[col < N ? col : n0];
+        }
+    }
+    float mk[2][2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) mk[i][j][r] = 1.f;
+    if (epi == EPI_RELU_MASK) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+                    const int col = n0 + wn + j * 16 + (lane & 15);
+                    mk[i][j][r] = mask[(size_t)(row < M ? row : m0) * ldc + (col < N ? col : n0)];
+                }
+    }
+    float out[2][2][4];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
-                const int col = n0 + wn + j * 16 + (lane & 15);
-                if (row >= M || col >= N) continue;
-                float v = acc[i][j][r];
-                if (epi == EPI_BIAS || epi == EPI_BIAS_RELU) v += bias[col];
+                float v = acc[i][j][r] + bj[j];  // bj = 0 without a bias
                 if (epi == EPI_BIAS_RELU) v = v > 0.f ? v : 0.f;
-                if (epi == EPI_RELU_MASK) v = mask[(size_t)row * ldc + col] > 0.f ? v : 0.f;
-                C[(size_t)row * ldc + col] = v;
+                if (epi == EPI_RELU_MASK) v = mk[i][j][r] > 0.f ? v : 0.f;
+                out[i][j][r] = v;
             }
+    // The values are pinned in registers here (an empty asm that takes them), so the arithmetic above, and
+    // its wait for the loaded operands, is not sunk into the per-element store branches below: a wait there
+    // would also wait for the stores issued before it.
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) asm volatile("" : "+v"(out[i][j][r]));
+    if (m0 + TM <= M && n0 + TN <= N) {  // a whole tile: the same for every lane of the workgroup
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+                    const int col = n0 + wn + j * 16 + (lane & 15);
+                    C[(size_t)row * ldc + col] = out[i][j][r];
+                }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+                    const int col = n0 + wn + j * 16 + (lane & 15);
+                    if (row < M && col < N) C[(size_t)row * ldc + col] = out[i][j][r];
+                }
+    }
 }
 
 // dY = 2 * mask * (Y - T) / max(count, 1); per-block partial sums of the loss
@@ -160,7 +283,18 @@ __global__ void reduce_partials(int nb, const float* __restrict__ partial, float
                                 float* __restrict__ loss_out) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     float l = 0.f, c = 0.f;
-    for (int i = 0; i < nb; ++i) {
+    int i = 0;
+    for (; i + 8 <= nb; i += 8) {  // eight pairs loaded, then added in order
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = partial[2 * i + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            l += v[2 * u];
+            c += v[2 * u + 1];
+        }
+    }
+    for (; i < nb; ++i) {
         l += partial[2 * i];
         c += partial[2 * i + 1];
     }
@@ -179,18 +313,37 @@ __global__ __launch_bounds__(256) void col_sum(int n, int cols, int rc, const fl
     const int r0 = blockIdx.y * rc;
     const int r1 = r0 + rc < n ? r0 + rc : n;
     float acc = 0.f;
-    if (col < cols)
-        for (int i = r0 + wave; i < r1; i += 4) acc += dZ[(size_t)i * cols + col];
+    if (col < cols) {
+        int i = r0 + wave;
+        for (; i + 28 < r1; i += 32) {  // eight rows loaded, then added in order
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = dZ[(size_t)(i + 4 * u) * cols + col];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += v[u];
+        }
+        for (; i < r1; i += 4) acc += dZ[(size_t)i * cols + col];
+    }
     s[wave][lane] = acc;
     __syncthreads();
     if (wave == 0 && col < cols) part[(size_t)blockIdx.y * cols + col] = (s[0][lane] + s[1][lane]) + (s[2][lane] + s[3][lane]);
 }
 
-// out[i] = sum over z = 0 .. nz-1 of part[z * n + i], in that order.
+// out[i] = sum over z = 0 .. nz-1 of part[z * n + i], in that order.  The
+// slices are loaded eight at a time before they are added, so one thread waits
+// for memory once per eight slices rather than once per slice.
 __global__ void sum_partials(int n, int nz, const float* __restrict__ part, float* __restrict__ out) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float acc = part[i];
-        for (int z = 1; z < nz; ++z) acc += part[(size_t)z * n + i];
+        int z = 1;
+        for (; z + 8 <= nz; z += 8) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = part[(size_t)(z + u) * n + i];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += v[u];
+        }
+        for (; z < nz; ++z) acc += part[(size_t)z * n + i];
         out[i] = acc;
     }
 }

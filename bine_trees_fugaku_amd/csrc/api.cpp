@@ -209,7 +209,10 @@ struct bine_vcomm_s {
 
 namespace {
 
-// Drop least recently used plans until at most `keep` remain (caller holds comm->mu).
+// Drop least recently used plans until at most `keep` remain (caller holds comm->mu).  A plan that a batch or
+// a call in progress still holds lives on until they drop it.  The last holder's release frees the device
+// arrays with hipFree, which synchronizes the device first, so a launch of an earlier call that is still
+// running finishes before its work list goes away.
 void evict_lru(bine_vcomm_t comm, size_t keep) {
     while (comm->plans.size() > keep) {
         auto victim = comm->plans.begin();

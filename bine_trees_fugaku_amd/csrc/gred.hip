@@ -3,9 +3,10 @@
 // A work item names a group of leaves (buffer, rank, offset) whose element-wise
 // reduction tree is perfect, a list of outputs, and a tile [delta, delta + n)
 // of elements.  A workgroup streams the tile: for each 16-byte vector it loads
-// the leaves in groups of eight, reduces each group pairwise in registers and
-// folds the group sums with a binary-counter stack, so the association is the
-// balanced tree ((l0+l1)+(l2+l3))+... over the leaf order the host chose.  The
+// the leaves in groups of four or eight, reduces each group pairwise in
+// registers and folds the group sums with a binary-counter stack, so the
+// association is the balanced tree ((l0+l1)+(l2+l3))+... over the leaf order
+// the host chose.  The
 // sum is then stored to every output.  One-leaf items are copies (permutes),
 // two-leaf items are the per-step "received + local" reductions.
 //

@@ -330,7 +330,8 @@ int bine_sim_classes(const bine_sim_case_t* c, uint32_t* nclasses, uint32_t* kin
 int bine_sim_peers(const bine_sim_case_t* c, uint32_t cls, uint32_t* dest);
 /* Upload n cases and allocate their link counters on `device`. */
 int bine_sim_create(const bine_sim_case_t* cases, int n, int device, bine_sim_t* out);
-/* Count every class of every case: one GPU pass of integer atomics. */
+/* Count every class of every case in one GPU pass: integer difference updates
+ * per dimension run of each message, then prefix sums along every ring. */
 int bine_sim_run(bine_sim_t sim, void* stream);
 int bine_sim_case_info(bine_sim_t sim, int i, uint32_t* nclasses, uint32_t* nlinks);
 /* Per class of case i (synchronous copies after the last run's stream): share, repetitions, the most

@@ -76,6 +76,17 @@ def time_launches(fn, stream, steps, warmup):
     return wall * 1e3, e0.elapsed_time(e1) / steps
 
 
+def max_ulp(a, b):
+    """Largest distance in units in the last place between two float64 tensors (0 when bit-identical): the
+    sign-magnitude bit patterns are mapped onto one monotone integer line and subtracted."""
+    import torch
+    lo = torch.iinfo(torch.int64).min
+    ia, ib = a.view(torch.int64), b.view(torch.int64)
+    ia = torch.where(ia < 0, lo - ia, ia)
+    ib = torch.where(ib < 0, lo - ib, ib)
+    return int((ia - ib).abs().max().item())
+
+
 def launch_spread(fn, stream, n):
     """Milliseconds of n launches timed one by one (HIP events around each, synchronised in between)."""
     import torch
@@ -554,6 +565,7 @@ def main():
         step.allreduce(send, recv_step, stream=stream)
     torch.cuda.synchronize()
     same = bool(torch.equal(recv.view(torch.int64), recv_step.view(torch.int64)))
+    ulp = max_ulp(recv, recv_step)
 
     # one timer for value, ms_per_step and roofline.achieved: the HIP events around the timed launches
     gbps = b_alg / (ev_ms * 1e-3) / 1e9
@@ -580,7 +592,7 @@ def main():
             "per_step_GBps": round(step_gbps, 2), "per_step_ms": round(s_ev, 4),
             "per_step_launches": sstats["launches"], "per_step_schedule_bytes": sstats["bytes_read"] + sstats["bytes_written"],
             "batched_vs_per_step": round(s_ev / ev_ms, 3), "pct_hbm_peak": round(100 * gbps / HBM_PEAK_GBPS, 2),
-            "batched_equals_per_step_bitwise": same,
+            "batched_equals_per_step_bitwise": same, "max_ulp_batched_vs_per_step": ulp,
             "device_copy_GBps": round(b_alg / (c_ev * 1e-3) / 1e9, 2),
         },
         "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",

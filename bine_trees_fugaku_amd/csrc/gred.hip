@@ -354,11 +354,19 @@ struct Pack {
     Vec<T> x[VPT];
 };
 
+// Leaves per group of tree_eval: eight where the registers allow (two vectors per thread, or four vectors with
+// a stack of six levels, trees of up to 512 leaves), four otherwise.
+template <int VPT, int MAXS>
+constexpr int leaf_group() {
+    return VPT == 4 && MAXS != 6 ? 4 : 8;
+}
+
 template <typename T, int VPT, int MAXS, bool BATCH, bool NT, bool REALIGN>
 __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ items, uint32_t nitems,
                                                    const PoolEntry* __restrict__ pool, BufTable tab,
                                                    const BufTable* __restrict__ tabs, uint32_t allow_vec) {
     constexpr int VE = Vec<T>::N;
+    constexpr int G = leaf_group<VPT, MAXS>();
     // partial subtree sums of the waves, [wave][u][lane] 16-byte vectors
     extern __shared__ __align__(16) unsigned char lds_raw[];
     Vec<T>* lds = reinterpret_cast<Vec<T>*>(lds_raw);
@@ -413,7 +421,7 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
                     }
                     return r;
                 };
-                Pack<T, VPT> part = tree_eval<Pack<T, VPT>, MAXS, (VPT == 4 ? 4 : 8)>(lgs, addp, loadp);
+                Pack<T, VPT> part = tree_eval<Pack<T, VPT>, MAXS, G>(lgs, addp, loadp);
 #pragma unroll
                 for (int u = 0; u < VPT; ++u) lds[((size_t)wave * VPT + u) * 64 + lane] = part.x[u];
                 __syncthreads();
@@ -484,7 +492,7 @@ __global__ void __launch_bounds__(512) gred_kernel(const GredItem* __restrict__ 
                     }
                     return r;
                 };
-                Pack<T, VPT> s = tree_eval<Pack<T, VPT>, MAXS, (VPT == 4 ? 4 : 8)>(lg, addp, loadp);
+                Pack<T, VPT> s = tree_eval<Pack<T, VPT>, MAXS, G>(lg, addp, loadp);
                 if (REALIGN) {
                     u4 v[VPT];
                     Prev rot[VPT];
@@ -597,11 +605,17 @@ static hipError_t launch_v(int max_lg, const GredItem* items, uint32_t n, const 
     case 2:
         return small ? launch_t<T, 2, 6, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st)
                      : launch_t<T, 2, 11, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st);
-    default:  // four vectors per thread: groups of 4 leaves, one more stack level; deep trees run with two
-        // trees of up to 64 leaves need four stack levels: 163 VGPRs, three waves per SIMD instead of two
+    default:  // four vectors per thread; deep trees run with two
+        // trees of up to 64 leaves: groups of 4 leaves, four stack levels, 163 VGPRs, three waves per SIMD
         if (max_lg <= 6) return launch_t<T, 4, 4, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st);
-        return small ? launch_t<T, 4, 7, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st)
-                     : launch_t<T, 2, 11, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st);
+        if (!small) return launch_t<T, 2, 11, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st);
+        // up to 512 leaves: groups of 8 leaves and six stack levels (250 VGPRs, no scratch; 0.3-1.4 % faster on
+        // the 512-rank allreduce than groups of 4 with seven levels), except where those registers do not fit:
+        // int8's byte arithmetic and the realigning stores
+        if constexpr (sizeof(T) > 1 && !RA)
+            return launch_t<T, 4, 6, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st);
+        else
+            return launch_t<T, 4, 7, BATCH, RA>(items, n, pool, tab, tabs, allow_vec, sh, st);
     }
 }
 

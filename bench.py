@@ -76,6 +76,32 @@ def time_launches(fn, stream, steps, warmup):
     return wall * 1e3, e0.elapsed_time(e1) / steps
 
 
+def time_gpu(fn, stream, steps, warmup, one_by_one=0):
+    """GPU milliseconds per launch with the host's enqueue time taken out: a short device-side spin (torch's
+    sleep kernel, on the current stream, which must be `stream`) runs first, so every launch of the timed
+    region is already queued when the GPU reaches the start event.  Returns the mean over `steps` back-to-back
+    launches and, when one_by_one > 0, that many launches timed singly the same way."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+
+    def timed(n):
+        torch.cuda._sleep(4_000_000)  # about 2 ms at 2.4 GHz: longer than enqueueing the launches below
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    mean = timed(steps)
+    singles = [timed(1) for _ in range(one_by_one)]
+    return mean, singles
+
+
 def max_ulp(a, b):
     """Largest distance in units in the last place between two float64 tensors (0 when bit-identical): the
     sign-magnitude bit patterns are mapped onto one monotone integer line and subtracted."""
@@ -85,21 +111,6 @@ def max_ulp(a, b):
     ia = torch.where(ia < 0, lo - ia, ia)
     ib = torch.where(ib < 0, lo - ib, ib)
     return int((ia - ib).abs().max().item())
-
-
-def launch_spread(fn, stream, n):
-    """Milliseconds of n launches timed one by one (HIP events around each, synchronised in between)."""
-    import torch
-    ts = []
-    for _ in range(n):
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        fn()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        ts.append(e0.elapsed_time(e1))
-    return ts
 
 
 def cpu_baseline(warmup, runs):
@@ -444,10 +455,10 @@ def per_collective_8x8x8(stream):
             else:
                 fn = lambda: comm.reduce(send, recv, stream=stream)  # noqa: E731
             comm.autotune(name, send, recv, count, exec="FUSED", iters=5, stream=stream)
-            w, ev = time_launches(fn, stream, 10, 2)
-            one = sorted(launch_spread(fn, stream, 9))
-            # an occasional launch runs 2-4x longer than the rest (seen on the alltoall); the mean of the timed
-            # launches keeps it, the median of the single launches shows the typical one beside it
+            # GPU time only: timed from events after a device-side spin, so that a slow enqueue on the host
+            # (the single-call path looks up its plan on every call) cannot leave the GPU idle inside the region
+            ev, one = time_gpu(fn, stream, 10, 2, one_by_one=9)
+            one = sorted(one)
             out[name] = {"algorithmic_bytes": int(b_alg), "ms": round(ev, 4),
                          "GBps": round(b_alg / (ev * 1e-3) / 1e9, 1),
                          "pct_hbm_peak": round(100 * b_alg / (ev * 1e-3) / 1e9 / HBM_PEAK_GBPS, 1),

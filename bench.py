@@ -77,9 +77,9 @@ def time_launches(fn, stream, steps, warmup):
 
 
 def time_gpu(fn, stream, steps, warmup, one_by_one=0):
-    """GPU milliseconds per launch with the host's enqueue time taken out: a short device-side spin (torch's
-    sleep kernel, on the current stream, which must be `stream`) runs first, so every launch of the timed
-    region is already queued when the GPU reaches the start event.  Returns the mean over `steps` back-to-back
+    """GPU milliseconds per launch with the host's issue time and the GPU's idle spells taken out: a short
+    device-side spin (torch's sleep kernel, on the current stream, which must be `stream`) runs first, so every
+    launch of the timed region is already queued, and the GPU busy, when it reaches the start event.  Returns the mean over `steps` back-to-back
     launches and, when one_by_one > 0, that many launches timed singly the same way."""
     import torch
     for _ in range(warmup):
@@ -455,8 +455,8 @@ def per_collective_8x8x8(stream):
             else:
                 fn = lambda: comm.reduce(send, recv, stream=stream)  # noqa: E731
             comm.autotune(name, send, recv, count, exec="FUSED", iters=5, stream=stream)
-            # GPU time only: timed from events after a device-side spin, so that a slow enqueue on the host
-            # (the single-call path looks up its plan on every call) cannot leave the GPU idle inside the region
+            # GPU time only: timed from events after a device-side spin, so that neither the host's issue time nor
+            # the GPU's resumption after an idle spell falls inside the region
             ev, one = time_gpu(fn, stream, 10, 2, one_by_one=9)
             one = sorted(one)
             out[name] = {"algorithmic_bytes": int(b_alg), "ms": round(ev, 4),

@@ -378,6 +378,12 @@ def small_configs(stream, cpu=True):
             res["batched_GBps"] = round(b_alg / (w * 1e-3) / 1e9, 1)
             w, _ = time_launches(lambda: bt.launch_graph(stream), stream, 20, 3)
             res["graph_replay_ms"] = round(w, 4)
+            # the same launches with the host's issue time taken out (all queued behind a device-side spin)
+            gpu_ms, _ = time_gpu(lambda: bt.launch(stream), stream, 20, 3)
+            res["batched_gpu_only_ms"] = round(gpu_ms, 4)
+            res["batched_gpu_only_GBps"] = round(b_alg / (gpu_ms * 1e-3) / 1e9, 1)
+            gpu_ms, _ = time_gpu(lambda: bt.launch_graph(stream), stream, 20, 3)
+            res["graph_replay_gpu_only_ms"] = round(gpu_ms, 4)
             bt.close()
             c.close()
             # the per-step schedule (one launch per step) issued directly and as one captured graph
@@ -387,6 +393,10 @@ def small_configs(stream, cpu=True):
             res["step_direct_ms"] = round(w, 4)
             w, _ = time_launches(lambda: bt.launch_graph(stream), stream, 20, 3)
             res["step_graph_ms"] = round(w, 4)
+            gpu_ms, _ = time_gpu(lambda: bt.launch(stream), stream, 20, 3)
+            res["step_direct_gpu_only_ms"] = round(gpu_ms, 4)
+            gpu_ms, _ = time_gpu(lambda: bt.launch_graph(stream), stream, 20, 3)
+            res["step_graph_gpu_only_ms"] = round(gpu_ms, 4)
             bt.close()
             c.close()
         torch.cuda.synchronize()

@@ -49,7 +49,7 @@ constexpr int SR = TM + 16;  // LDS row stride of a panel held [k][row]
 // running along X's contiguous dimension, so a whole panel moves as 16-byte
 // loads and 16-byte LDS stores.  LDS holds the panel in X's own orientation:
 // [r][k] with row stride SK when KC, [k][r] with row stride SR otherwise; both
-// strides make the fragment reads of the products free of bank conflicts.
+// strides spread a fragment read's 64 lanes over 64 consecutive banks.
 template <bool KC>
 struct Panel {
     const float* X;
@@ -150,7 +150,8 @@ __global__ __launch_bounds__(256) void gemm_f32(int M, int N, int K, int kc, con
             pb.fetch(vb, tid, k0 + TK, kend, kbeg);
         }
         // 16x16x4: lane supplies A[row = lane % 16][k = lane / 16] and B[k = lane / 16][col = lane % 16].
-        // The whole chunk's fragments are read from LDS first, so the reads wait once, not per product.
+        // The chunk's fragments go to registers ahead of the products that use them, so LDS latency overlaps
+        // the matrix cores.
         float af[TK / 4][2], bf[TK / 4][2];
 #pragma unroll
         for (int q = 0; q < TK / 4; ++q) {

@@ -21,7 +21,7 @@ EXPORTED = [
     "bine_vcomm_size", "bine_vcomm_set_exec", "bine_vcomm_set_cache_limit", "bine_vcomm_trim",
     "bine_vcomm_cache_info", "bine_allreduce", "bine_allreduce_noalloc", "bine_reduce_scatter", "bine_allgather",
     "bine_bcast", "bine_alltoall", "bine_scatter", "bine_gather", "bine_reduce", "bine_compute_peers",
-    "bine_compute_step_info", "bine_block_step", "bine_get_tree", "bine_remap_rank", "bine_reduce_local",
+    "bine_compute_step_info", "bine_get_peer", "bine_get_mirroring_port", "bine_block_step", "bine_get_tree", "bine_remap_rank", "bine_reduce_local",
     "bine_reduce_local3",    "bine_plan_stats", "bine_plan_autotune", "bine_plan_dump", "bine_batch_create", "bine_batch_launch", "bine_batch_launch_graph",
     "bine_batch_autotune", "bine_batch_stats", "bine_batch_get_tuning", "bine_batch_set_tuning",
     "bine_batch_destroy", "bine_tuning_candidates", "bine_batch_time_shapes", "bine_sim_classes", "bine_sim_peers", "bine_sim_create", "bine_sim_run", "bine_sim_case_info",
@@ -98,6 +98,8 @@ def _sig(L):
     L.bine_gather.argtypes = [vp, i32, i32, vp, i32, i32, i32, vp, vp]
     L.bine_reduce.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp]
     L.bine_compute_peers.argtypes = [pu32, u32, u32, i32, i32, pu32]
+    L.bine_get_peer.argtypes = [pu32, u32, ctypes.POINTER(ctypes.c_int32), u32, u32, i32, ctypes.POINTER(ctypes.c_int32)]
+    L.bine_get_mirroring_port.argtypes = [i32, u32]
     L.bine_compute_step_info.argtypes = [pu32, u32, u32, pu32, pu32]
     L.bine_block_step.argtypes = [pu32, u32, u32, u32, i32, pu32]
     L.bine_get_tree.argtypes = [pu32, u32, u32, u32, i32, i32, pu32, pu32, pu32, pu32, pu32]

@@ -948,6 +948,30 @@ extern "C" int bine_compute_step_info(const uint32_t* dims, uint32_t ndims, uint
     return BINE_OK;
 }
 
+extern "C" int bine_get_peer(const uint32_t* dims, uint32_t ndims, const int32_t* coord, uint32_t step, uint32_t port,
+                             int family, int32_t* coord_peer) {
+    Torus t;
+    if (!dims || !coord || !coord_peer || !Torus::make(dims, ndims, &t) || step >= t.steps)
+        return fail(BINE_ERR_ARG, "bad arguments");
+    if (family != FAM_BINE && family != FAM_RECDOUB) return fail(BINE_ERR_ARG, "family must be BINE or RECDOUB");
+    int c[kMaxDims] = {0, 0, 0};
+    for (uint32_t i = 0; i < t.nd; ++i) {
+        if (coord[i] < 0 || coord[i] >= (int32_t)t.dims[i]) return fail(BINE_ERR_ARG, "coordinate out of range");
+        c[i] = coord[i];
+    }
+    const uint32_t peer = peer_of(t, c, step, port, step_dims(t, port), family);
+    t.coords(peer, c);
+    for (uint32_t i = 0; i < t.nd; ++i) coord_peer[i] = c[i];
+    return BINE_OK;
+}
+
+extern "C" int bine_get_mirroring_port(int num_ports, uint32_t ndims) {
+    if (ndims < 1 || ndims > kMaxDims) return -1;
+    for (int p = 0; p < num_ports; ++p)
+        if (mirrored_port(p, ndims)) return p;
+    return -1;
+}
+
 extern "C" int bine_block_step(const uint32_t* dims, uint32_t ndims, uint32_t rank, uint32_t port, int family,
                                uint32_t* block_step) {
     Torus t;

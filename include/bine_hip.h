@@ -177,6 +177,13 @@ int bine_reduce(const void* sendbuf, void* recvbuf, int count, bine_datatype_t d
 int bine_compute_peers(const uint32_t* dims, uint32_t ndims, uint32_t rank, int port, int family, uint32_t* peers);
 /* compute_step_info, lib/libbine_coll.cc:306: d[num_steps], step_in_d[num_steps] */
 int bine_compute_step_info(const uint32_t* dims, uint32_t ndims, uint32_t port, uint32_t* d, uint32_t* step_in_d);
+/* get_peer_c, lib/libbine_coll.cc:89-107: the coordinates of coord's peer at schedule step `step` of `port`
+ * (coord and coord_peer hold ndims entries) */
+int bine_get_peer(const uint32_t* dims, uint32_t ndims, const int32_t* coord, uint32_t step, uint32_t port,
+                  int family, int32_t* coord_peer);
+/* get_mirroring_port, lib/libbine_coll.cc:44-52: the first of num_ports ports whose schedule is mirrored
+ * (distances of opposite sign), or -1 */
+int bine_get_mirroring_port(int num_ports, uint32_t ndims);
 /* BineBitmapCalculator::block_step, lib/libbine_common.cc:1491-1510, 2036-2045: block_step[P] */
 int bine_block_step(const uint32_t* dims, uint32_t ndims, uint32_t rank, uint32_t port, int family,
                     uint32_t* block_step);

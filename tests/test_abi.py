@@ -1,5 +1,7 @@
 """The C-ABI library loads on the CPU and exports every symbol include/bine_hip.h declares."""
 import ctypes
+
+import numpy as np
 import os
 import re
 
@@ -79,6 +81,61 @@ def test_schedule_helpers_match_oracle():
                     bs = (ctypes.c_uint32 * P)()
                     _lib.check(L.bine_block_step(d, len(dims), r, port, 1, bs), "block_step")
                     assert list(bs) == list(O.block_step(dims, r, port))
+
+
+def test_get_peer_and_mirroring_port():
+    """get_peer_c (lib/libbine_coll.cc:89-107): one step from a coordinate.  Expected values from the definition:
+    step i moves dimension d of compute_step_info's (d, k) by rho_k = (1 - (-2)^(k+1)) / 3 for BINE (negated on an
+    odd coordinate) or to c ^ 2^k for RECDOUB, negated again on a mirrored port; on tori of equal dimensions the
+    result is compute_peers' peer of the rank.  get_mirroring_port (lib/libbine_coll.cc:44-52): the first mirrored
+    port, ports >= 3 on 3D tori, 2-3 on 2D, odd ones on 1D."""
+    from oracle import oracle as O
+    L = _lib.load()
+    i32 = ctypes.c_int32
+
+    def mirrored(port, nd):
+        return port >= 3 if nd == 3 else (port in (2, 3) if nd == 2 else port % 2 == 1)
+
+    for dims in ([8], [16], [4, 4], [6, 6], [8, 8, 8], [2, 8, 2], [4, 16]):
+        nd = len(dims)
+        d = (ctypes.c_uint32 * 3)(*(list(dims) + [1] * (3 - nd)))
+        P = int(np.prod(dims))
+        for family, fam_code in (("BINE", 1), ("RECDOUB", 2)):
+            if family == "RECDOUB" and any(x & (x - 1) for x in dims):
+                continue
+            for port in range(2 * nd):
+                steps = O.step_info(dims, port)
+                for r in (range(P) if P <= 64 else [0, 1, 5, 77, P - 1]):
+                    coord = []
+                    x = r
+                    for n in reversed(dims):
+                        coord.append(x % n)
+                        x //= n
+                    coord = coord[::-1]
+                    peers = O.compute_peers(dims, r, port, family) if len(set(dims)) == 1 else None
+                    for i, (dd, k) in enumerate(steps):
+                        c = coord[dd]
+                        if family == "BINE":
+                            dist = (1 - (-2) ** (k + 1)) // 3 * (-1 if c % 2 else 1)
+                        else:
+                            dist = (c ^ (1 << k)) - c
+                        if mirrored(port, nd):
+                            dist = -dist
+                        want = list(coord)
+                        want[dd] = (c + dist) % dims[dd]
+                        out = (i32 * nd)()
+                        _lib.check(L.bine_get_peer(d, nd, (i32 * nd)(*coord), i, port, fam_code, out), "get_peer")
+                        assert list(out) == want, (dims, family, port, r, i)
+                        if peers is not None:
+                            rid = 0
+                            for n, x in zip(dims, want):
+                                rid = rid * n + x
+                            assert rid == peers[i], (dims, family, port, r, i)
+    assert L.bine_get_peer(d, 3, (i32 * 3)(8, 0, 0), 0, 0, 1, (i32 * 3)()) == _lib.BINE_ERR_ARG
+    for nd in (1, 2, 3):
+        for n in range(0, 7):
+            first = next((p for p in range(n) if mirrored(p, nd)), -1)
+            assert L.bine_get_mirroring_port(n, nd) == first, (n, nd)
 
 
 def test_trees_and_remap_match_oracle():

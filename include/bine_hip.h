@@ -182,7 +182,7 @@ int bine_compute_step_info(const uint32_t* dims, uint32_t ndims, uint32_t port, 
 int bine_get_peer(const uint32_t* dims, uint32_t ndims, const int32_t* coord, uint32_t step, uint32_t port,
                   int family, int32_t* coord_peer);
 /* get_mirroring_port, lib/libbine_coll.cc:44-52: the first of num_ports ports whose schedule is mirrored
- * (distances of opposite sign), or -1 */
+ * (distances of opposite sign), or -1 (also for ndims outside 1..3) */
 int bine_get_mirroring_port(int num_ports, uint32_t ndims);
 /* BineBitmapCalculator::block_step, lib/libbine_common.cc:1491-1510, 2036-2045: block_step[P] */
 int bine_block_step(const uint32_t* dims, uint32_t ndims, uint32_t rank, uint32_t port, int family,

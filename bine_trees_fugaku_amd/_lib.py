@@ -110,7 +110,7 @@ def _sig(L):
     L.bine_plan_stats.argtypes = [vp, i32, i32, i32, i32, i32, i32, ctypes.POINTER(PlanStats)]
     L.bine_plan_autotune.argtypes = [vp, i32, vp, vp, i32, i32, i32, i32, i32, vp, i32]
     L.bine_plan_dump.argtypes = [ctypes.POINTER(VCommConfig), i32, i32, i32, i32, i32, i32, i32, ctypes.c_int64,
-                                 ctypes.POINTER(PlanDumpInfo), vp, vp, pu32]
+                                 i32, ctypes.POINTER(PlanDumpInfo), vp, vp, pu32]
     L.bine_batch_create.argtypes = [vp, ctypes.POINTER(BatchItem), i32, ctypes.POINTER(vp)]
     L.bine_batch_launch.argtypes = [vp, vp]
     L.bine_batch_launch_graph.argtypes = [vp, vp]

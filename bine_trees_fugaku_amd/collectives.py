@@ -284,16 +284,17 @@ def _dtype_code(dtype) -> int:
 
 
 def plan_dump(dims: Sequence[int], coll: str, count: int, dtype, op: str = "SUM", root: int = 0, exec: str = "AUTO",
-              num_ports: int = 1, tile_elems: int = 0, in_place: bool = False, **overrides):
+              num_ports: int = 1, tile_elems: int = 0, in_place: bool = False, two_phase: int = 0, **overrides):
     """Compile a schedule on the host (no GPU) and return its lowered work as
     numpy arrays: (info dict, items structured array, pool structured array,
-    launches (n, 2) array).  in_place: the allreduce schedule for sendbuf == recvbuf."""
+    launches (n, 2) array).  in_place: the allreduce schedule for sendbuf == recvbuf.
+    two_phase: 1 or 2 lowers reductions with several outputs as two launches (see Batch.set_tuning)."""
     import numpy as np
     L = _lib.load()
     cfg = make_config(dims, num_ports=num_ports, **overrides)
     info = _lib.PlanDumpInfo()
     args = (ctypes.byref(cfg), COLL[coll], count, _dtype_code(dtype), OP[op], root, 1 if in_place else 0, EXEC[exec],
-            tile_elems, ctypes.byref(info))
+            tile_elems, two_phase, ctypes.byref(info))
     _lib.check(L.bine_plan_dump(*args, None, None, None), "bine_plan_dump")
     item_dt = np.dtype([("leaf_off", "<u4"), ("out_off", "<u4"), ("lg", "<u2"), ("flags", "<u2"), ("nouts", "<u4"),
                         ("delta", "<u4"), ("n", "<u4"), ("bucket", "<u4"), ("pad", "<u4")])
@@ -368,19 +369,23 @@ class Batch:
     def autotune(self, iters: int = 3, stream=None):
         _lib.check(self._lib.bine_batch_autotune(self.handle, VComm._stream(stream), iters), "bine_batch_autotune")
 
-    def set_tuning(self, vpt: int, threads: int, persistent: int = 0, nontemporal: int = 0):
-        _lib.check(self._lib.bine_batch_set_tuning(self.handle, vpt, threads, persistent, nontemporal),
+    def set_tuning(self, vpt: int, threads: int, persistent: int = 0, nontemporal: int = 0, two_phase: int = 0):
+        """two_phase: 0 one launch per list; 1 reductions with several outputs write their first output, and a
+        second launch copies it to the others (copies in group order); 2 the same, copies tile by tile."""
+        flags = (1 if nontemporal else 0) | (int(two_phase) << 1)
+        _lib.check(self._lib.bine_batch_set_tuning(self.handle, vpt, threads, persistent, flags),
                    "bine_batch_set_tuning")
 
     def tuning(self) -> dict:
-        v, t, p, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        v, t, p, f = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _lib.check(self._lib.bine_batch_get_tuning(self.handle, ctypes.byref(v), ctypes.byref(t), ctypes.byref(p),
-                                                   ctypes.byref(n)), "bine_batch_get_tuning")
-        return {"vpt": v.value, "threads": t.value, "persistent_wgs": p.value, "nontemporal": n.value}
+                                                   ctypes.byref(f)), "bine_batch_get_tuning")
+        return {"vpt": v.value, "threads": t.value, "persistent_wgs": p.value, "nontemporal": f.value & 1,
+                "two_phase": f.value >> 1}
 
     def time_shapes(self, shapes, iters: int = 3, stream=None):
         """Average ms of one launch at each launch shape (rows of vpt, threads,
-        persistent, nontemporal); the batch keeps its current shape."""
+        persistent, flags: bit 0 non-temporal, bits 1-2 the two-phase mode); the batch keeps its current shape."""
         import numpy as np
         arr = np.ascontiguousarray(np.asarray(shapes, np.int32).reshape(-1, 4))
         ms = np.zeros(len(arr), np.float32)

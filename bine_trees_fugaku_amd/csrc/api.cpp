@@ -134,7 +134,8 @@ int max_lg_of(const std::vector<std::vector<Group>>& gs) {
 int upload(const std::vector<std::vector<Group>>& gs, size_t ds, const LaunchShape& sh, uint32_t bucket, DevArrays* d,
            Retired* retire = nullptr) {
     Lowered low;
-    lower_groups(gs, shallow_tile_elems(sh, ds), (int)(16 / ds), bucket, ds, &low, deep_tile_elems(sh, ds), split_lg(sh));
+    lower_groups(gs, shallow_tile_elems(sh, ds), (int)(16 / ds), bucket, ds, &low, deep_tile_elems(sh, ds), split_lg(sh),
+                 nullptr, nullptr, sh.two_phase);
     return install(std::move(low), max_lg_of(gs), shallow_tile_elems(sh, ds), sh, d, retire);
 }
 
@@ -798,6 +799,22 @@ std::vector<LaunchShape> tuning_candidates(int cus) {
             {4, 128, 8 * cus, 1}, {4, 512, 0, 1}, {4, 512, 4 * cus, 1}};
 }
 
+// The two-phase trials of an autotune: the three fastest one-launch shapes, each with its copies in group order
+// and tile by tile.  Which order wins depends on the list (measured: group order for 512-leaf trees of two tiles,
+// tile by tile for 64-leaf trees of many tiles), and neither wins everywhere.
+std::vector<LaunchShape> two_phase_trials(std::vector<std::pair<float, LaunchShape>> timed) {
+    std::stable_sort(timed.begin(), timed.end(),
+                     [](const std::pair<float, LaunchShape>& a, const std::pair<float, LaunchShape>& b) { return a.first < b.first; });
+    std::vector<LaunchShape> out;
+    for (size_t i = 0; i < timed.size() && i < 3; ++i)
+        for (int m = 1; m <= 2; ++m) {
+            LaunchShape c = timed[i].second;
+            c.two_phase = m;
+            out.push_back(c);
+        }
+    return out;
+}
+
 // Issue one call of a collective (argument layout of its entry point).
 int issue(bine_vcomm_t comm, int32_t coll, const void* send, void* recv, int count, int dtype, int op, int root,
           void* stream) {
@@ -853,24 +870,38 @@ extern "C" int bine_plan_autotune(bine_vcomm_t comm, int32_t coll, const void* s
     LaunchShape bestc = d.shape;
     rc = BINE_OK;
     bool first = true;
-    for (const LaunchShape& c : cands) {
+    std::vector<std::pair<float, LaunchShape>> timed;
+    auto trial = [&](const LaunchShape& c) {
         // the arrays in place before the call may be named by captured graphs: they are kept, not freed;
         // the candidates' own arrays never leave this loop
-        if ((rc = upload(groups, dp->ds, c, 0, &d, first ? &dp->retired : nullptr))) break;
+        if ((rc = upload(groups, dp->ds, c, 0, &d, first ? &dp->retired : nullptr))) return;
         first = false;
-        if ((rc = issue(comm, coll, sendbuf, recvbuf, count, datatype, op, root, stream))) break;
-        if (hipEventRecord(e0, st) != hipSuccess) { rc = fail(BINE_ERR_HIP, "event"); break; }
+        if ((rc = issue(comm, coll, sendbuf, recvbuf, count, datatype, op, root, stream))) return;
+        if (hipEventRecord(e0, st) != hipSuccess) {
+            rc = fail(BINE_ERR_HIP, "event");
+            return;
+        }
         for (int i = 0; i < iters && !rc; ++i) rc = issue(comm, coll, sendbuf, recvbuf, count, datatype, op, root, stream);
-        if (rc) break;
+        if (rc) return;
         (void)hipEventRecord(e1, st);
         (void)hipEventSynchronize(e1);
         float ms = 0;
         (void)hipEventElapsedTime(&ms, e0, e1);
+        timed.emplace_back(ms, c);
         if (ms < best) {
             best = ms;
             bestc = c;
         }
+    };
+    for (const LaunchShape& c : cands) {
+        trial(c);
+        if (rc) break;
     }
+    if (!rc && has_multi_output_reduction(groups))
+        for (const LaunchShape& c : two_phase_trials(timed)) {
+            trial(c);
+            if (rc) break;
+        }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     comm->cfg.exec = saved_exec;
@@ -880,8 +911,10 @@ extern "C" int bine_plan_autotune(bine_vcomm_t comm, int32_t coll, const void* s
 
 extern "C" int bine_plan_dump(const bine_vcomm_config_t* cfg, int32_t coll, int count, bine_datatype_t datatype,
                               bine_op_t op, int root, int32_t in_place, int32_t exec, int64_t tile_elems,
-                              bine_plan_dump_info_t* info, void* items, void* pool, uint32_t* launches) {
+                              int32_t two_phase, bine_plan_dump_info_t* info, void* items, void* pool,
+                              uint32_t* launches) {
     if (!cfg || !info) return fail(BINE_ERR_ARG, "NULL argument");
+    if (two_phase < 0 || two_phase > 2) return fail(BINE_ERR_ARG, "two_phase must be 0, 1 or 2");
     const size_t ds = dtsize(datatype);
     if (!ds || count < 0) return fail(BINE_ERR_ARG, "bad datatype or count");
     if (cfg->num_ports < 1 || cfg->num_ports > (uint32_t)kMaxPorts) return fail(BINE_ERR_ARG, "num_ports must be 1..6");
@@ -902,8 +935,9 @@ extern "C" int bine_plan_dump(const bine_vcomm_config_t* cfg, int32_t coll, int 
     else deep = tile_elems;
     Lowered low;
     if (exec == BINE_EXEC_FUSED)
-        lower_groups(std::vector<std::vector<Group>>(1, cp.fused), tile_elems, (int)(16 / ds), 0, ds, &low, deep, split_lg(sh));
-    else lower_groups(cp.step_groups, tile_elems, (int)(16 / ds), 0, ds, &low, deep, split_lg(sh));
+        lower_groups(std::vector<std::vector<Group>>(1, cp.fused), tile_elems, (int)(16 / ds), 0, ds, &low, deep, split_lg(sh),
+                     nullptr, nullptr, two_phase);
+    else lower_groups(cp.step_groups, tile_elems, (int)(16 / ds), 0, ds, &low, deep, split_lg(sh), nullptr, nullptr, two_phase);
     info->n_items = low.items.size();
     info->n_pool = low.pool.size();
     info->n_launches = low.launches.size();
@@ -1082,7 +1116,7 @@ int lower_list(bine_batch_t b, bine_batch_s::TypeList& l) {
         std::vector<uint8_t> bucket_vec(b->tabs.size());
         for (size_t i = 0; i < b->tabs.size(); ++i) bucket_vec[i] = (uint8_t)vec_ok(b->tabs[i]);
         lower_groups(l.groups, shallow_tile_elems(l.shape, ds), (int)(16 / ds), 0, ds, &all, deep_tile_elems(l.shape, ds),
-                     split_lg(l.shape), &l.group_bucket, &bucket_vec);
+                     split_lg(l.shape), &l.group_bucket, &bucket_vec, l.shape.two_phase);
         int max_lg = max_lg_of(l.groups);
         bool tables_off16 = false;
         for (uint32_t gb : l.group_bucket) tables_off16 = tables_off16 || off16(b->tabs[gb]);
@@ -1102,19 +1136,21 @@ int batch_lower(bine_batch_t b) {
     return BINE_OK;
 }
 
-// Launch one element type's work list: the reduction kernel, then the copy runs.
+// Launch one element type's work list: per launch of the list, the reduction kernel, then the copy runs.
 int run_list(bine_batch_t b, const bine_batch_s::TypeList& l, hipStream_t st) {
-    if (l.arr.low.items.empty()) return BINE_OK;
-    const uint32_t nrun = l.arr.low.runs[0], nred = (uint32_t)l.arr.low.items.size() - nrun;
-    if (nred) {
-        hipError_t e = launch_gred(l.dtype, l.arr.max_lg, l.arr.items, nred, l.arr.pool, BufTable{}, b->d_tabs, true,
-                                   l.allow_vec, l.shape, st);
-        if (e != hipSuccess) return fail(BINE_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
-    }
-    if (nrun) {
-        hipError_t e = launch_grun((uint32_t)dtsize(l.dtype), l.arr.items + nred, nrun, l.arr.pool, BufTable{},
-                                   b->d_tabs, true, l.allow_vec, l.shape, st);
-        if (e != hipSuccess) return fail(BINE_ERR_HIP, std::string("batch copy-run launch: ") + hipGetErrorString(e));
+    const Lowered& low = l.arr.low;
+    for (size_t li = 0; li < low.launches.size(); ++li) {
+        const uint32_t first = low.launches[li].first, nrun = low.runs[li], nred = low.launches[li].second - nrun;
+        if (nred) {
+            hipError_t e = launch_gred(l.dtype, l.arr.max_lg, l.arr.items + first, nred, l.arr.pool, BufTable{},
+                                       b->d_tabs, true, l.allow_vec, l.shape, st);
+            if (e != hipSuccess) return fail(BINE_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
+        }
+        if (nrun) {
+            hipError_t e = launch_grun((uint32_t)dtsize(l.dtype), l.arr.items + first + nred, nrun, l.arr.pool,
+                                       BufTable{}, b->d_tabs, true, l.allow_vec, l.shape, st);
+            if (e != hipSuccess) return fail(BINE_ERR_HIP, std::string("batch copy-run launch: ") + hipGetErrorString(e));
+        }
     }
     return BINE_OK;
 }
@@ -1243,12 +1279,12 @@ extern "C" int bine_batch_stats(bine_batch_t b, bine_plan_stats_t* out) {
 }
 
 extern "C" int bine_batch_get_tuning(bine_batch_t b, int32_t* vpt, int32_t* threads, int32_t* persistent_wgs,
-                                     int32_t* nontemporal) {
+                                     int32_t* flags) {
     if (!b) return fail(BINE_ERR_ARG, "NULL batch");
     if (vpt) *vpt = b->shape.vpt;
     if (threads) *threads = b->shape.threads;
     if (persistent_wgs) *persistent_wgs = b->shape.persistent;
-    if (nontemporal) *nontemporal = b->shape.nontemporal;
+    if (flags) *flags = shape_flags(b->shape);
     return BINE_OK;
 }
 
@@ -1262,16 +1298,16 @@ void drop_graph(bine_batch_t b) {
 }  // namespace
 
 extern "C" int bine_batch_set_tuning(bine_batch_t b, int32_t vpt, int32_t threads, int32_t persistent_wgs,
-                                     int32_t nontemporal) {
+                                     int32_t flags) {
     if (!b) return fail(BINE_ERR_ARG, "NULL batch");
     if ((vpt != 1 && vpt != 2 && vpt != 4) || threads < 64 || threads > 512 || (threads & (threads - 1)) ||
-        persistent_wgs < 0)
+        persistent_wgs < 0 || flags < 0 || flags > 5)
         return fail(BINE_ERR_ARG, "bad launch shape");
-    LaunchShape sh{vpt, threads, persistent_wgs, nontemporal ? 1 : 0};
+    const LaunchShape sh = shape_from_words(vpt, threads, persistent_wgs, flags);
     b->shape = sh;
     drop_graph(b);
     for (auto& l : b->lists) {
-        const bool relower = sh.vpt != l.shape.vpt || sh.threads != l.shape.threads;
+        const bool relower = sh.vpt != l.shape.vpt || sh.threads != l.shape.threads || sh.two_phase != l.shape.two_phase;
         l.shape = sh;
         if (relower) {
             int rc = lower_list(b, l);
@@ -1303,7 +1339,7 @@ int time_shapes(bine_batch_t b, hipStream_t st, int iters, const std::vector<Lau
     HIPCHK(hipEventCreate(&ev.b));
     for (size_t i = 0; i < shapes.size(); ++i) {
         const LaunchShape& c = shapes[i];
-        int rc = bine_batch_set_tuning(b, c.vpt, c.threads, c.persistent, c.nontemporal);
+        int rc = bine_batch_set_tuning(b, c.vpt, c.threads, c.persistent, shape_flags(c));
         if (rc) return rc;
         if ((rc = batch_run(b, st))) return rc;  // warm-up
         HIPCHK(hipEventRecord(ev.a, st));
@@ -1328,7 +1364,7 @@ extern "C" int bine_tuning_candidates(int device, int32_t* shapes, int cap, int*
         shapes[4 * i] = c[i].vpt;
         shapes[4 * i + 1] = c[i].threads;
         shapes[4 * i + 2] = c[i].persistent;
-        shapes[4 * i + 3] = c[i].nontemporal;
+        shapes[4 * i + 3] = shape_flags(c[i]);
     }
     return BINE_OK;
 }
@@ -1337,11 +1373,11 @@ extern "C" int bine_batch_time_shapes(bine_batch_t b, void* stream, int iters, c
                                       float* ms) {
     if (!b || iters < 1 || n < 1 || !shapes || !ms) return fail(BINE_ERR_ARG, "bad arguments");
     std::vector<LaunchShape> v;
-    for (int i = 0; i < n; ++i) v.push_back(LaunchShape{shapes[4 * i], shapes[4 * i + 1], shapes[4 * i + 2], shapes[4 * i + 3]});
+    for (int i = 0; i < n; ++i) v.push_back(shape_from_words(shapes[4 * i], shapes[4 * i + 1], shapes[4 * i + 2], shapes[4 * i + 3]));
     HIPCHK(hipSetDevice(b->comm->cfg.device));
     const LaunchShape keep = b->shape;
     int rc = time_shapes(b, (hipStream_t)stream, iters, v, ms);
-    int rc2 = bine_batch_set_tuning(b, keep.vpt, keep.threads, keep.persistent, keep.nontemporal);
+    int rc2 = bine_batch_set_tuning(b, keep.vpt, keep.threads, keep.persistent, shape_flags(keep));
     return rc ? rc : rc2;
 }
 
@@ -1358,31 +1394,40 @@ extern "C" int bine_batch_autotune(bine_batch_t b, void* stream, int iters) {
     HIPCHK(hipEventCreate(&ev.a));
     HIPCHK(hipEventCreate(&ev.b));
     drop_graph(b);
-    for (const LaunchShape& c : cands) {
-        for (size_t i = 0; i < nl; ++i) {
-            auto& l = b->lists[i];
-            const bool relower = c.vpt != l.shape.vpt || c.threads != l.shape.threads;
-            l.shape = c;
-            int rc = relower ? lower_list(b, l) : BINE_OK;
-            if (rc) return rc;
-            if ((rc = run_list(b, l, st))) return rc;  // warm-up
-            HIPCHK(hipEventRecord(ev.a, st));
-            for (int k = 0; k < iters; ++k)
-                if ((rc = run_list(b, l, st))) return rc;
-            HIPCHK(hipEventRecord(ev.b, st));
-            HIPCHK(hipEventSynchronize(ev.b));
-            float t = 0;
-            HIPCHK(hipEventElapsedTime(&t, ev.a, ev.b));
-            if (t < best[i]) {
-                best[i] = t;
-                pick[i] = c;
-            }
+    std::vector<std::vector<std::pair<float, LaunchShape>>> timed(nl);
+    auto trial = [&](size_t i, const LaunchShape& c) -> int {
+        auto& l = b->lists[i];
+        const bool relower = c.vpt != l.shape.vpt || c.threads != l.shape.threads || c.two_phase != l.shape.two_phase;
+        l.shape = c;
+        int rc = relower ? lower_list(b, l) : BINE_OK;
+        if (rc) return rc;
+        if ((rc = run_list(b, l, st))) return rc;  // warm-up
+        HIPCHK(hipEventRecord(ev.a, st));
+        for (int k = 0; k < iters; ++k)
+            if ((rc = run_list(b, l, st))) return rc;
+        HIPCHK(hipEventRecord(ev.b, st));
+        HIPCHK(hipEventSynchronize(ev.b));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, ev.a, ev.b));
+        timed[i].emplace_back(t, c);
+        if (t < best[i]) {
+            best[i] = t;
+            pick[i] = c;
         }
-    }
+        return BINE_OK;
+    };
+    for (const LaunchShape& c : cands)
+        for (size_t i = 0; i < nl; ++i)
+            if (int rc = trial(i, c)) return rc;
+    for (size_t i = 0; i < nl; ++i)
+        if (has_multi_output_reduction(b->lists[i].groups))
+            for (const LaunchShape& c : two_phase_trials(timed[i]))
+                if (int rc = trial(i, c)) return rc;
     size_t dominant = 0;
     for (size_t i = 0; i < nl; ++i) {
         auto& l = b->lists[i];
-        const bool relower = pick[i].vpt != l.shape.vpt || pick[i].threads != l.shape.threads;
+        const bool relower = pick[i].vpt != l.shape.vpt || pick[i].threads != l.shape.threads ||
+                             pick[i].two_phase != l.shape.two_phase;
         l.shape = pick[i];
         if (relower) {
             int rc = lower_list(b, l);

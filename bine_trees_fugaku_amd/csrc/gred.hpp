@@ -53,7 +53,16 @@ struct LaunchShape {
     int threads = 256;    // workgroup size: 64..512 (a multiple of 64)
     int persistent = 0;   // 0: one workgroup per item; >0: grid of that many workgroups
     int nontemporal = 0;  // streaming loads/stores
+    int two_phase = 0;    // reductions with several outputs: 0 one launch; 1 two launches (the reductions
+                          // write their first output, then copies of it fill the others), copies in group
+                          // order; 2 the same, copies tile by tile
 };
+
+// The fourth launch-shape word of the C interface: bit 0 streaming, bits 1-2 the two-phase mode.
+inline LaunchShape shape_from_words(int vpt, int threads, int persistent, int flags) {
+    return LaunchShape{vpt, threads, persistent, flags & 1, (flags >> 1) & 3};
+}
+inline int shape_flags(const LaunchShape& s) { return (s.nontemporal ? 1 : 0) | (s.two_phase << 1); }
 
 // Items whose tree has at least 2^(log2(waves) + 3) leaves are reduced with
 // their leaves split across the waves of the workgroup; their tile is one

@@ -1702,8 +1702,8 @@ bool compile_program(const Program& prog, bool want_fused, CompiledPlan* out, st
 
 void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shallow_tile, int vec, uint32_t bucket,
                   size_t dtsize, Lowered* out, int64_t deep_tile_elems, int deep_lg,
-                  const std::vector<uint32_t>* group_bucket, const std::vector<uint8_t>* bucket_vec) {
-    for (const auto& gs : launches) {
+                  const std::vector<uint32_t>* group_bucket, const std::vector<uint8_t>* bucket_vec, int two_phase) {
+    auto lower_launch = [&](const std::vector<Group>& gs, const std::vector<uint32_t>* gbk, bool copies_by_tile) {
         const uint32_t first = (uint32_t)out->items.size();
         // Open run of aligned single copies: consecutive groups whose leaf and
         // output advance by one fixed (rank, offset) step become one ITEM_RUN item.
@@ -1730,7 +1730,7 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
         };
         for (size_t gi = 0; gi < gs.size(); ++gi) {
             const Group& g = gs[gi];
-            const uint32_t gb = group_bucket ? (*group_bucket)[gi] : bucket;
+            const uint32_t gb = gbk ? (*gbk)[gi] : bucket;
             if (g.n <= 0) continue;
             if (single_copy(g, gb) && run_item >= 0 && run_len < kMaxRun && out->items[run_item].n == (uint32_t)g.n &&
                 out->items[run_item].bucket == gb) {
@@ -1854,7 +1854,7 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
             if (ra != rb) return rb;
             const bool ca = ia.lg == 0, cb = ib.lg == 0;
             if (ca != cb) return cb;
-            if (ca) return false;
+            if (ca && !copies_by_tile) return false;
             if (ia.bucket != ib.bucket) return ia.bucket < ib.bucket;
             return tile_no[a] < tile_no[b];
         });
@@ -1876,7 +1876,47 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
         }
         out->launches.push_back({first, (uint32_t)cnt});
         out->runs.push_back(nrun);
+    };
+    for (size_t li = 0; li < launches.size(); ++li) {
+        const std::vector<Group>& gs = launches[li];
+        const std::vector<uint32_t>* gb = li == 0 ? group_bucket : nullptr;
+        bool split = false;
+        for (const Group& g : gs) split = split || (two_phase != 0 && g.lg > 0 && g.outs.size() > 1);
+        if (!split) {
+            lower_launch(gs, gb, false);
+            continue;
+        }
+        // Two phases: each reduction writes its first output only, and a second launch copies that output to
+        // the others.  The first launch is all reads (the leaves), the second all writes (the copies), so the
+        // memory does not alternate between the two directions.  Same bits: the copies move the sums unchanged.
+        std::vector<Group> red, bc;
+        std::vector<uint32_t> red_b, bc_b;
+        for (size_t gi = 0; gi < gs.size(); ++gi) {
+            const Group& g = gs[gi];
+            const uint32_t b = gb ? (*gb)[gi] : bucket;
+            red.push_back(g);
+            red_b.push_back(b);
+            if (g.lg > 0 && g.outs.size() > 1) {
+                red.back().outs.resize(1);
+                Group c;
+                c.n = g.n;
+                c.lg = 0;
+                c.leaves.assign(1, g.outs[0]);
+                c.outs.assign(g.outs.begin() + 1, g.outs.end());
+                bc.push_back(std::move(c));
+                bc_b.push_back(b);
+            }
+        }
+        lower_launch(red, &red_b, false);
+        lower_launch(bc, &bc_b, two_phase == 2);
     }
+}
+
+bool has_multi_output_reduction(const std::vector<std::vector<Group>>& launches) {
+    for (const auto& gs : launches)
+        for (const Group& g : gs)
+            if (g.lg > 0 && g.outs.size() > 1) return true;
+    return false;
 }
 
 }  // namespace bine

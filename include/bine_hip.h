@@ -226,7 +226,9 @@ int bine_plan_stats(bine_vcomm_t comm, int32_t coll, int count, bine_datatype_t 
 /* Time candidate launch shapes (vectors per thread, workgroup size, persistent
  * grid, streaming) of one collective's cached schedule on the caller's buffers
  * and keep the fastest for later calls.  Runs the collective (iters + 1) times
- * per candidate (bine_tuning_candidates lists them; 28 on an MI355X).  The
+ * per candidate (bine_tuning_candidates lists them; 28 on an MI355X), then,
+ * where the schedule holds reductions with several outputs, at the three
+ * fastest shapes in both two-phase modes (six more; see bine_batch_set_tuning).  The
  * work list in place before the call stays allocated until the vcomm is
  * destroyed, so graphs captured earlier keep running the old launch shape.
  * Must not run concurrently with other calls on the same vcomm. */
@@ -238,7 +240,8 @@ int bine_plan_autotune(bine_vcomm_t comm, int32_t coll, const void* sendbuf, voi
  * arrays to size them.  items: info->n_items records of 32 bytes (bine
  * gather-reduce items); pool: info->n_pool records of 8 bytes; launches:
  * info->n_launches pairs (first item, item count).  in_place: the allreduce
- * schedule for sendbuf == recvbuf (SEND has no extent).  No HIP call is made. */
+ * schedule for sendbuf == recvbuf (SEND has no extent).  two_phase: 0, or the
+ * two-phase mode 1 or 2 of bine_batch_set_tuning.  No HIP call is made. */
 typedef struct {
     uint64_t n_items, n_pool, n_launches;
     int64_t extent[4];  /* per-rank elements of SEND, RECV, TMP, TMP2 */
@@ -247,8 +250,8 @@ typedef struct {
     int32_t fused_ok;
 } bine_plan_dump_info_t;
 int bine_plan_dump(const bine_vcomm_config_t* cfg, int32_t coll, int count, bine_datatype_t datatype, bine_op_t op,
-                   int root, int32_t in_place, int32_t exec, int64_t tile_elems, bine_plan_dump_info_t* info,
-                   void* items, void* pool, uint32_t* launches);
+                   int root, int32_t in_place, int32_t exec, int64_t tile_elems, int32_t two_phase,
+                   bine_plan_dump_info_t* info, void* items, void* pool, uint32_t* launches);
 
 /* ---- batching: many collectives in one launch (no counterpart in the reference:
  * it replaces the per-call read_env/BlockInfo/tmpbuf set-up, lib/libbine.cc:1013-1050,
@@ -272,25 +275,32 @@ typedef struct bine_batch_s* bine_batch_t;
  * replaces its work lists: graphs the caller captured from bine_batch_launch
  * before that must be captured again (bine_batch_launch_graph does so itself). */
 int bine_batch_create(bine_vcomm_t comm, const bine_batch_item_t* items, int n, bine_batch_t* out);
-/* Run the whole batch: one persistent launch per element type present. */
+/* Run the whole batch: one launch per element type present (two for a list tuned to two phases). */
 int bine_batch_launch(bine_batch_t batch, void* stream);
 /* Same work captured once into a HIP graph and replayed. */
 int bine_batch_launch_graph(bine_batch_t batch, void* stream);
 /* Time candidate (vector width, workgroup, persistence, streaming) shapes and keep the fastest, chosen
- * separately for each element type's work list. */
+ * separately for each element type's work list; a list holding reductions with several outputs is also
+ * timed at its three fastest shapes in both two-phase modes. */
 int bine_batch_autotune(bine_batch_t batch, void* stream, int iters);
 int bine_batch_stats(bine_batch_t batch, bine_plan_stats_t* out);
 /* Launch shape: 16-byte vectors per thread (1, 2, 4), threads per workgroup (64..512, power of two),
- * persistent grid (0 = one workgroup per item), streaming (non-temporal) loads and stores.  get returns
- * the last shape set, or after autotuning the shape of the list that takes longest; set applies one
- * shape to every list. */
+ * persistent grid (0 = one workgroup per item), flags: bit 0 (BINE_TUNE_NONTEMPORAL) streaming loads and
+ * stores; bits 1-2 the two-phase mode for reductions with several outputs (an allreduce's blocks):
+ * 0 one launch, the reductions writing every output; BINE_TUNE_TWO_PHASE (2) the reductions write their
+ * first output, then a second launch copies it to the others, so the memory serves a read phase and a
+ * write phase instead of both at once (same bits); BINE_TUNE_TWO_PHASE_BY_TILE (4) the same with the
+ * copies issued tile by tile across the blocks instead of block by block.  get returns the last shape
+ * set, or after autotuning the shape of the list that takes longest; set applies one shape to every list. */
+enum { BINE_TUNE_NONTEMPORAL = 1, BINE_TUNE_TWO_PHASE = 2, BINE_TUNE_TWO_PHASE_BY_TILE = 4 };
 int bine_batch_get_tuning(bine_batch_t batch, int32_t* vpt, int32_t* threads, int32_t* persistent_wgs,
-                          int32_t* nontemporal);
+                          int32_t* flags);
 int bine_batch_set_tuning(bine_batch_t batch, int32_t vpt, int32_t threads, int32_t persistent_wgs,
-                          int32_t nontemporal);
+                          int32_t flags);
 int bine_batch_destroy(bine_batch_t batch);
 /* The autotuner's candidate launch shapes on `device`: up to cap rows of
- * {vectors per thread, threads, persistent workgroups, non-temporal}; *n = total. */
+ * {vectors per thread, threads, persistent workgroups, flags}; *n = total.  The
+ * two-phase trials an autotune adds are not listed. */
 int bine_tuning_candidates(int device, int32_t* shapes, int cap, int* n);
 /* Average time (ms) of one batch launch at each of n shapes (rows as above):
  * one warm-up launch, then iters timed ones.  The batch keeps its shape. */

@@ -355,3 +355,51 @@ def test_family_default_runs_its_stand_in(coll, dims, standin):
     assert a[0] == b[0]
     for x, y in zip(a[1:], b[1:]):
         assert x.tobytes() == y.tobytes()
+
+
+# ---- two-phase lowering ----------------------------------------------------------------
+
+@pytest.mark.parametrize("dims,algo,ports", [([8], "B", 1), ([4, 4], "B", 1), ([4, 4, 4], "B_CONT", 1), ([2, 8, 2], "B", 1),
+                                             ([4, 4], "B", 2), ([2, 2, 2], "B_COALESCE", 3), ([8, 8, 8], "B", 1)])
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("dtype", ["float32", "float64", "int32"])
+def test_two_phase_allreduce(dims, algo, ports, mode, dtype):
+    """Two-phase lowering of the fused allreduce (the schedules that fuse: powers of two; L and non-powers of two
+    run per step): the first launch holds the reductions, each writing one output;
+    the second holds copies of that output to the block's other P - 1 outputs (group by group in mode 1, tile by
+    tile in mode 2).  Every input is read once, every output written once, and each block's sum once more read
+    back for the copies.  The emulated result equals the oracle bit for bit."""
+    P = int(np.prod(dims))
+    count = 5 * P * ports + 3
+    x = data((P, count), dtype, P + mode)
+    kw = dict(allreduce_algo=algo, num_ports=ports)
+    meta, items, pool, launches = plan_dump(dims, "allreduce", count, dtype, exec="FUSED", tile_elems=16,
+                                            two_phase=mode, **kw)
+    assert len(launches) == 2
+    (f1, n1), (f2, n2) = launches
+    first, second = items[f1:f1 + n1], items[f2:f2 + n2]
+    red = first[first["lg"] > 0]
+    assert len(red) > 0 and (red["nouts"] == 1).all()
+    assert (second["lg"] == 0).all() and (second["nouts"] == P - 1).all()
+    reads = sum(int(it["n"]) << int(it["lg"]) for it in items)
+    writes = sum(int(it["n"]) * int(it["nouts"]) for it in items)
+    assert reads == (P + 1) * count and writes == P * count
+    if mode == 1:  # group by group: one block's copies are consecutive
+        assert (np.diff(second["leaf_off"].astype(np.int64)) >= 0).all()
+    else:  # the same copies in another order
+        m1 = plan_dump(dims, "allreduce", count, dtype, exec="FUSED", tile_elems=16, two_phase=1, **kw)
+        other = m1[1][m1[3][1][0]:]
+        assert sorted(r.tobytes() for r in second) == sorted(r.tobytes() for r in other)
+    _, bufs = run_plan(dims, "allreduce", count, dtype, "FUSED", send=x, tile=16, two_phase=mode, **kw)
+    assert same_bits(bufs[E.BUF_RECV], O.allreduce(x, dims, algo=algo, ports=ports))
+
+
+@pytest.mark.parametrize("coll", ["reduce_scatter", "reduce", "allgather", "bcast"])
+def test_two_phase_leaves_other_lists_alone(coll):
+    """Lists without a reduction of several outputs lower the same with or without two phases."""
+    dims, count = [4, 4], 64
+    kw = {"root": 0} if coll in ("reduce", "bcast") else {}
+    a = plan_dump(dims, coll, count, "float32", exec="FUSED", **kw)
+    b = plan_dump(dims, coll, count, "float32", exec="FUSED", two_phase=2, **kw)
+    assert len(a[3]) == len(b[3]) == 1
+    assert a[1].tobytes() == b[1].tobytes() and a[2].tobytes() == b[2].tobytes()

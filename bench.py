@@ -7,9 +7,12 @@ allreduce of all 512 ranks.  The timed region is bracketed by device
 synchronisations; `value`, `ms_per_step` and `roofline.achieved` all come from
 HIP events recorded on the launch stream around it (the wall clock of the same
 region is reported beside them).  The measured path is the batched execution: the
-whole Bine schedule compiled into one fused gather-reduce launch (replaying
+whole Bine schedule compiled into one fused gather-reduce work list (replaying
 the reference's reduction tree for every element), launch shape picked by the
-autotuner, issued by bine_batch_launch.  The same batch replayed as a captured
+autotuner, issued by bine_batch_launch.  The autotuner may keep the two-phase
+lowering: one launch reduces every block into one rank's output, a second
+copies it to the other ranks' outputs, so HBM serves a read phase and then a
+write phase (same bits).  The same batch replayed as a captured
 HIP graph and the plain per-step path (one kernel per schedule step, 1 + 9 + 9
 launches) are timed beside it.
 
@@ -54,7 +57,8 @@ def parse():
     ap.add_argument("--cpu-runs", type=int, default=10, help="timed oracle runs of the CPU baseline")
     ap.add_argument("--only", choices=["batched", "step", "copy", "alltoall", "buckets", "c2"], default=None,
                     help="profiling aid: run only that part, --steps times, with a fixed launch shape")
-    ap.add_argument("--tuning", default="2,128,0,0", help="vpt,threads,persistent,nontemporal for --only batched/alltoall/buckets")
+    ap.add_argument("--tuning", default="2,128,0,0",
+                    help="vpt,threads,persistent,nontemporal[,two_phase] for --only batched/alltoall/buckets/c2")
     return ap.parse_args()
 
 
@@ -138,6 +142,12 @@ def cpu_baseline(warmup, runs):
                       f"{float(np.median(per)):.3f} s, {threads} OpenMP threads; GB/s = 2*P*S / mean time"}
 
 
+def tuning_words(text):
+    """vpt,threads,persistent,nontemporal[,two_phase] -> the arguments of Batch.set_tuning."""
+    w = [int(v) for v in text.split(",")]
+    return w[:4] + [w[4] if len(w) > 4 else 0]
+
+
 def profile_only(args, stream, send, recv, recv_step):
     """Run one part of the benchmark for a profiler: the batched fused launch
     with a fixed shape, the per-step launches, a plain device copy of the
@@ -150,8 +160,7 @@ def profile_only(args, stream, send, recv, recv_step):
         if args.only == "batched":
             comm = VComm(DIMS, exec="FUSED")
             batch = Batch(comm, [{"coll": "allreduce", "sendbuf": send, "recvbuf": recv, "count": COUNT}])
-            vpt, thr, pers, nt = (int(v) for v in args.tuning.split(","))
-            batch.set_tuning(vpt, thr, pers, nt)
+            batch.set_tuning(*tuning_words(args.tuning))
             for _ in range(args.steps):
                 batch.launch(stream)
         elif args.only == "alltoall":
@@ -161,8 +170,7 @@ def profile_only(args, stream, send, recv, recv_step):
             b = torch.empty_like(a)
             comm = VComm(DIMS, exec="FUSED")
             batch = Batch(comm, [{"coll": "alltoall", "sendbuf": a, "recvbuf": b, "count": 512}])
-            vpt, thr, pers, nt = (int(v) for v in args.tuning.split(","))
-            batch.set_tuning(vpt, thr, pers, nt)
+            batch.set_tuning(*tuning_words(args.tuning))
             for _ in range(args.steps):
                 batch.launch(stream)
         elif args.only == "buckets":
@@ -176,8 +184,7 @@ def profile_only(args, stream, send, recv, recv_step):
             comm = VComm([4, 4, 4], exec="FUSED")
             batch = Batch(comm, [{"coll": "allreduce", "sendbuf": x, "recvbuf": y, "count": int(n)}
                                  for x, y, n in zip(sends, recvs, sizes)])
-            vpt, thr, pers, nt = (int(v) for v in args.tuning.split(","))
-            batch.set_tuning(vpt, thr, pers, nt)
+            batch.set_tuning(*tuning_words(args.tuning))
             for _ in range(args.steps):
                 batch.launch(stream)
         elif args.only == "c2":
@@ -187,8 +194,7 @@ def profile_only(args, stream, send, recv, recv_step):
             y = torch.empty_like(x)
             comm = VComm([4, 4, 4], exec="FUSED")
             batch = Batch(comm, [{"coll": "allreduce", "sendbuf": x, "recvbuf": y, "count": 4194304}])
-            vpt, thr, pers, nt = (int(v) for v in args.tuning.split(","))
-            batch.set_tuning(vpt, thr, pers, nt)
+            batch.set_tuning(*tuning_words(args.tuning))
             for _ in range(args.steps):
                 batch.launch(stream)
         elif args.only == "step":
@@ -596,7 +602,7 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic_allreduce_8x8x8_fp64.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            traffic = json.load(f).get("hbm_bytes_per_step")
 
     line = {
         "metric": METRIC, "value": round(gbps, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -605,7 +611,8 @@ def main():
         "config": {
             "workload": "allreduce_8x8x8_fp64_4MiB_per_rank", "P": P, "torus": "8x8x8", "elements_per_rank": COUNT,
             "bytes_per_rank": COUNT * ELEM, "algorithmic_bytes": b_alg,
-            "exec": "batched: one fused persistent launch (bine_batch_launch)",
+            "exec": "batched: the autotuned fused work list (bine_batch_launch): one launch, or two when the "
+                    "two-phase lowering wins (reductions writing one output per block, then copies to the others)",
             "timer": "HIP events on the launch stream around the timed launches, synchronised on both sides",
             "wall_ms_per_step": round(wall_ms, 4),
             "graph_replay_ms": round(g_ev, 4), "graph_replay_GBps": round(b_alg / (g_ev * 1e-3) / 1e9, 2),

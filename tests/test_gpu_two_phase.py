@@ -139,3 +139,28 @@ def test_autotune_with_two_phase_trials(gpu):
         torch.cuda.synchronize()
         assert same_bits(recv, ref), tun
         b.close()
+
+
+@pytest.mark.parametrize("dims,count,dt", [([4, 4, 4], 64 * 4096 + 7, torch.float32), ([8, 8, 8], 512 * 256, torch.float64)])
+def test_single_call_autotune_with_two_phase_trials(gpu, dims, count, dt):
+    """bine_plan_autotune on a fused allreduce also times the two-phase trials; the call it leaves behind writes
+    the bits it wrote before tuning, and the oracle's on a sampled block."""
+    P = int(np.prod(dims))
+    send = randn((P, count), dt, 21)
+    recv = torch.empty_like(send)
+    with VComm(dims, exec="FUSED") as comm:
+        comm.allreduce(send, recv)
+        torch.cuda.synchronize()
+        ref = recv.clone()
+        comm.autotune("allreduce", send, recv, count, exec="FUSED", iters=1)
+        recv.zero_()
+        comm.allreduce(send, recv)
+        torch.cuda.synchronize()
+        assert same_bits(recv, ref)
+    cols = send[:, :P].contiguous()  # a P-element allreduce: one element per block, against the oracle
+    out = torch.empty_like(cols)
+    with VComm(dims, exec="FUSED") as comm:
+        comm.autotune("allreduce", cols, out, P, exec="FUSED", iters=1)
+        comm.allreduce(cols, out)
+        torch.cuda.synchronize()
+        assert out.cpu().numpy().tobytes() == O.allreduce(cols.cpu().numpy(), dims).tobytes()

@@ -1,6 +1,7 @@
 """Seeded random batches on the GPU: each batch holds two to six collectives of different kinds, element types,
 counts and roots on one virtual communicator (an in-place allreduce now and then), is launched directly, replayed
-as a captured graph and retuned, and every item's output is compared with the oracle bit for bit after each.
+as a captured graph, retuned and run in two phases, and every item's output is compared with the oracle bit for
+bit after each.
 
 Rooted reductions on a multi-dimensional torus use int32 (exact whatever the tree); copies (bcast, scatter,
 gather, allgather, alltoall) have a unique result."""
@@ -145,5 +146,9 @@ def test_random_batch(gpu, seed):
             reset(items, bufs)
             bt.launch()
             check(items, bufs, name, "retuned launch")
+            reset(items, bufs)
+            bt.set_tuning(2, 128, 0, 1, two_phase=1 + seed % 2)  # reductions, then copies of their first output
+            bt.launch()
+            check(items, bufs, name, "two-phase launch")
         finally:
             bt.close()

@@ -185,6 +185,23 @@ struct PlanKeyHash {
 LaunchShape default_shape() { return LaunchShape{1, 256, 0, 0}; }
 int64_t default_tile_elems(size_t ds) { return shallow_tile_elems(default_shape(), ds); }
 
+// The shape a fused work list starts with before any tuning.  Lists of at least 256 MiB whose trees have 64
+// leaves or more start at four vectors per thread, one-wave workgroups and streaming loads and stores, the
+// one-launch shape the autotuner keeps on them (untuned, the general shape moved the 512-rank fp64 allreduce of
+// 4 MiB per rank at 4.5 TB/s and the 64-rank fp32 one of 16 MiB per rank at 5.0).  Every other list starts at the
+// general shape.
+LaunchShape initial_shape(const std::vector<std::vector<Group>>& gs, size_t ds) {
+    uint64_t bytes = 0;
+    int lg = 0;
+    for (const auto& v : gs)
+        for (const Group& g : v) {
+            bytes += (uint64_t)g.n * (g.leaves.size() + g.outs.size()) * ds;
+            lg = std::max<int>(lg, g.lg);
+        }
+    if (bytes >= (256ull << 20) && lg >= 6) return LaunchShape{4, 64, 0, 1};
+    return default_shape();
+}
+
 }  // namespace
 
 // Error text shared with the other C-ABI translation units (sim, selector).
@@ -257,7 +274,7 @@ int get_plan(bine_vcomm_t comm, const CollRequest& req, int dtype, std::shared_p
     }
     if (dp->cp.fused_ok) {
         std::vector<std::vector<Group>> one(1, dp->cp.fused);
-        rc = upload(one, ds, default_shape(), 0, &dp->fused);
+        rc = upload(one, ds, initial_shape(one, ds), 0, &dp->fused);
         if (rc) return rc;
     }
     // temporaries are allocated only when some step touches them
@@ -1224,6 +1241,8 @@ extern "C" int bine_batch_create(bine_vcomm_t comm, const bine_batch_item_t* ite
     HIPCHK(hipSetDevice(comm->cfg.device));
     HIPCHK(hipMalloc(&b->d_tabs, sizeof(BufTable) * b->tabs.size()));
     HIPCHK(hipMemcpy(b->d_tabs, b->tabs.data(), sizeof(BufTable) * b->tabs.size(), hipMemcpyHostToDevice));
+    for (auto& l : b->lists) l.shape = initial_shape(l.groups, dtsize(l.dtype));
+    if (!b->lists.empty()) b->shape = b->lists[0].shape;
     int rc = batch_lower(b.get());
     if (rc) return rc;
     *out = b.release();

@@ -53,6 +53,8 @@ def make_config(dims: Sequence[int], num_ports: int = 1, exec: str = "AUTO", dev
     LIBBINE_* overlay, then explicit overrides such as ``allreduce_algo="B_CONT"``,
     ``alltoall_family="BRUCK"`` or ``allgather_distance="INCREASING"``."""
     L = _lib.load()
+    if not 1 <= len(dims) <= 3:  # LIBBINE_MAX_SUPPORTED_DIMENSIONS (lib/libbine_common.h:19)
+        raise ValueError(f"a torus has 1 to 3 dimensions, got {len(dims)}")
     cfg = _lib.VCommConfig()
     size = 1
     for d in dims:

@@ -185,20 +185,28 @@ struct PlanKeyHash {
 LaunchShape default_shape() { return LaunchShape{1, 256, 0, 0}; }
 int64_t default_tile_elems(size_t ds) { return shallow_tile_elems(default_shape(), ds); }
 
-// The shape a fused work list starts with before any tuning.  Lists of at least 256 MiB whose trees have 64
-// leaves or more start at four vectors per thread, one-wave workgroups and streaming loads and stores, the
-// one-launch shape the autotuner keeps on them (untuned, the general shape moved the 512-rank fp64 allreduce of
-// 4 MiB per rank at 4.5 TB/s and the 64-rank fp32 one of 16 MiB per rank at 5.0).  Every other list starts at the
-// general shape.
+// The shape a fused work list starts with before any tuning.  Four vectors per thread, one-wave workgroups and
+// streaming loads and stores (the one-launch shape the autotuner keeps on long blocks) when the reductions of 64
+// leaves or more hold at least 90 % of the list's bytes in blocks of two whole tiles or more at that shape, and
+// those blocks give at least 1024 tiles, so every CU gets several.  Untuned, that shape moved the 512-rank fp64
+// allreduce of 4 MiB per rank at 6.1 TB/s against 4.5 with the general shape; on shorter blocks or fewer tiles
+// (the same torus at 256 KiB per rank, or 64 ranks of 2 MiB) it ran 8-70 % slower than the general shape, which
+// every other list starts at.
 LaunchShape initial_shape(const std::vector<std::vector<Group>>& gs, size_t ds) {
-    uint64_t bytes = 0;
-    int lg = 0;
+    const LaunchShape wide{4, 64, 0, 1};
+    const int64_t tile = deep_tile_elems(wide, ds);
+    uint64_t bytes = 0, long_bytes = 0, tiles = 0;
     for (const auto& v : gs)
         for (const Group& g : v) {
-            bytes += (uint64_t)g.n * (g.leaves.size() + g.outs.size()) * ds;
-            lg = std::max<int>(lg, g.lg);
+            if (g.lg == 0) continue;
+            const uint64_t b = (uint64_t)g.n * (g.leaves.size() + g.outs.size()) * ds;
+            bytes += b;
+            if (g.lg >= 6 && g.n >= 2 * tile) {
+                long_bytes += b;
+                tiles += (uint64_t)(g.n / tile);
+            }
         }
-    if (bytes >= (256ull << 20) && lg >= 6) return LaunchShape{4, 64, 0, 1};
+    if (tiles >= 1024 && long_bytes * 10 >= bytes * 9) return wide;
     return default_shape();
 }
 

@@ -292,8 +292,9 @@ int bine_batch_stats(bine_batch_t batch, bine_plan_stats_t* out);
  * write phase instead of both at once (same bits); BINE_TUNE_TWO_PHASE_BY_TILE (4) the same with the
  * copies issued tile by tile across the blocks instead of block by block.  get returns the last shape
  * set, or after autotuning the shape of the list that takes longest, or before either the first list's
- * starting shape (4, 64, 0, streaming for lists of 256 MiB or more with trees of 64 leaves or more;
- * 1, 256, 0, 0 otherwise); set applies one shape to every list. */
+ * starting shape (4, 64, 0, streaming where reductions of 64 leaves or more over blocks of two whole
+ * tiles or more hold 90 % of the list's bytes in at least 1024 tiles; 1, 256, 0, 0 otherwise); set
+ * applies one shape to every list. */
 enum { BINE_TUNE_NONTEMPORAL = 1, BINE_TUNE_TWO_PHASE = 2, BINE_TUNE_TWO_PHASE_BY_TILE = 4 };
 int bine_batch_get_tuning(bine_batch_t batch, int32_t* vpt, int32_t* threads, int32_t* persistent_wgs,
                           int32_t* flags);

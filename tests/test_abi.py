@@ -181,6 +181,22 @@ def test_remap_rank_export_against_golden_fixture():
         assert L.bine_remap_rank(r, P) == want, (P, r)
 
 
+def test_torus_dimensions_are_one_to_three():
+    """LIBBINE_MAX_SUPPORTED_DIMENSIONS is 3 (lib/libbine_common.h:19): a fourth dimension is refused on both
+    sides of the boundary, in Python before a config is built and in C by plan_dump."""
+    from bine_trees_fugaku_amd import make_config
+    with pytest.raises(ValueError):
+        make_config([4, 4, 4, 4])
+    with pytest.raises(ValueError):
+        make_config([])
+    cfg = make_config([4, 4, 4])
+    cfg.dimensions_num = 4
+    info = _lib.PlanDumpInfo()
+    L = _lib.load()
+    assert L.bine_plan_dump(ctypes.byref(cfg), 0, 256, 2, 0, 0, 0, 1, 0, 0, ctypes.byref(info), None, None,
+                            None) == _lib.BINE_ERR_ARG
+
+
 def test_host_only_entry_points_check_their_arguments():
     """Argument errors of the simulator, tuning and network entry points are
     reported before any device call (so they run without a GPU)."""

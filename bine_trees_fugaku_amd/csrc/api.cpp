@@ -190,7 +190,7 @@ int64_t default_tile_elems(size_t ds) { return shallow_tile_elems(default_shape(
 // leaves or more hold at least 90 % of the list's bytes in blocks of two whole tiles or more at that shape, and
 // those blocks give at least 1024 tiles, so every CU gets several.  Untuned, that shape moved the 512-rank fp64
 // allreduce of 4 MiB per rank at 6.1 TB/s against 4.5 with the general shape; on shorter blocks or fewer tiles
-// (the same torus at 256 KiB per rank, or 64 ranks of 2 MiB) it ran 8-70 % slower than the general shape, which
+// (the same torus at 256 KiB per rank, or 64 ranks of 2 MiB) it ran 10-67 % slower than the general shape, which
 // every other list starts at.
 LaunchShape initial_shape(const std::vector<std::vector<Group>>& gs, size_t ds) {
     const LaunchShape wide{4, 64, 0, 1};

@@ -290,7 +290,8 @@ def plan_dump(dims: Sequence[int], coll: str, count: int, dtype, op: str = "SUM"
     """Compile a schedule on the host (no GPU) and return its lowered work as
     numpy arrays: (info dict, items structured array, pool structured array,
     launches (n, 2) array).  in_place: the allreduce schedule for sendbuf == recvbuf.
-    two_phase: 1 or 2 lowers reductions with several outputs as two launches (see Batch.set_tuning)."""
+    two_phase: 1 or 2 lowers reductions with several outputs as two launches (see Batch.set_tuning); adding 4
+    gives the reduction launch items of two tiles."""
     import numpy as np
     L = _lib.load()
     cfg = make_config(dims, num_ports=num_ports, **overrides)
@@ -371,10 +372,12 @@ class Batch:
     def autotune(self, iters: int = 3, stream=None):
         _lib.check(self._lib.bine_batch_autotune(self.handle, VComm._stream(stream), iters), "bine_batch_autotune")
 
-    def set_tuning(self, vpt: int, threads: int, persistent: int = 0, nontemporal: int = 0, two_phase: int = 0):
+    def set_tuning(self, vpt: int, threads: int, persistent: int = 0, nontemporal: int = 0, two_phase: int = 0,
+                   double_tiles: int = 0):
         """two_phase: 0 one launch per list; 1 reductions with several outputs write their first output, and a
-        second launch copies it to the others (copies in group order); 2 the same, copies tile by tile."""
-        flags = (1 if nontemporal else 0) | (int(two_phase) << 1)
+        second launch copies it to the others (copies in group order); 2 the same, copies tile by tile.
+        double_tiles: with two phases, reduction items of two tiles (half as many items)."""
+        flags = (1 if nontemporal else 0) | (int(two_phase) << 1) | (8 if double_tiles else 0)
         _lib.check(self._lib.bine_batch_set_tuning(self.handle, vpt, threads, persistent, flags),
                    "bine_batch_set_tuning")
 
@@ -383,11 +386,12 @@ class Batch:
         _lib.check(self._lib.bine_batch_get_tuning(self.handle, ctypes.byref(v), ctypes.byref(t), ctypes.byref(p),
                                                    ctypes.byref(f)), "bine_batch_get_tuning")
         return {"vpt": v.value, "threads": t.value, "persistent_wgs": p.value, "nontemporal": f.value & 1,
-                "two_phase": f.value >> 1}
+                "two_phase": (f.value >> 1) & 3, "double_tiles": (f.value >> 3) & 1}
 
     def time_shapes(self, shapes, iters: int = 3, stream=None):
         """Average ms of one launch at each launch shape (rows of vpt, threads,
-        persistent, flags: bit 0 non-temporal, bits 1-2 the two-phase mode); the batch keeps its current shape."""
+        persistent, flags: bit 0 non-temporal, bits 1-2 the two-phase mode, bit 3 reduction items of two tiles);
+        the batch keeps its current shape."""
         import numpy as np
         arr = np.ascontiguousarray(np.asarray(shapes, np.int32).reshape(-1, 4))
         ms = np.zeros(len(arr), np.float32)

@@ -135,7 +135,7 @@ int upload(const std::vector<std::vector<Group>>& gs, size_t ds, const LaunchSha
            Retired* retire = nullptr) {
     Lowered low;
     lower_groups(gs, shallow_tile_elems(sh, ds), (int)(16 / ds), bucket, ds, &low, deep_tile_elems(sh, ds), split_lg(sh),
-                 nullptr, nullptr, sh.two_phase);
+                 nullptr, nullptr, sh.two_phase, sh.double_tiles ? 2 : 1);
     return install(std::move(low), max_lg_of(gs), shallow_tile_elems(sh, ds), sh, d, retire);
 }
 
@@ -825,18 +825,22 @@ std::vector<LaunchShape> tuning_candidates(int cus) {
 }
 
 // The two-phase trials of an autotune: the three fastest one-launch shapes, each with its copies in group order
-// and tile by tile.  Which order wins depends on the list (measured: group order for 512-leaf trees of two tiles,
-// tile by tile for 64-leaf trees of many tiles), and neither wins everywhere.
+// and tile by tile, and with reduction items of one and of two tiles.  Which order wins depends on the list
+// (measured: group order for 512-leaf trees of two tiles, tile by tile for 64-leaf trees of many tiles), and so
+// does the item length (two tiles: 1-2.5 % faster on 64 ranks of 64 tiles per block, 10 % slower on 512 ranks of
+// two), so none of them wins everywhere.
 std::vector<LaunchShape> two_phase_trials(std::vector<std::pair<float, LaunchShape>> timed) {
     std::stable_sort(timed.begin(), timed.end(),
                      [](const std::pair<float, LaunchShape>& a, const std::pair<float, LaunchShape>& b) { return a.first < b.first; });
     std::vector<LaunchShape> out;
     for (size_t i = 0; i < timed.size() && i < 3; ++i)
-        for (int m = 1; m <= 2; ++m) {
-            LaunchShape c = timed[i].second;
-            c.two_phase = m;
-            out.push_back(c);
-        }
+        for (int m = 1; m <= 2; ++m)
+            for (int d = 0; d <= 1; ++d) {
+                LaunchShape c = timed[i].second;
+                c.two_phase = m;
+                c.double_tiles = d;
+                out.push_back(c);
+            }
     return out;
 }
 
@@ -939,7 +943,8 @@ extern "C" int bine_plan_dump(const bine_vcomm_config_t* cfg, int32_t coll, int 
                               int32_t two_phase, bine_plan_dump_info_t* info, void* items, void* pool,
                               uint32_t* launches) {
     if (!cfg || !info) return fail(BINE_ERR_ARG, "NULL argument");
-    if (two_phase < 0 || two_phase > 2) return fail(BINE_ERR_ARG, "two_phase must be 0, 1 or 2");
+    if (two_phase < 0 || two_phase > 7 || (two_phase & 3) == 3)
+        return fail(BINE_ERR_ARG, "two_phase must be 0, 1 or 2, plus 4 for reduction items of two tiles");
     const size_t ds = dtsize(datatype);
     if (!ds || count < 0) return fail(BINE_ERR_ARG, "bad datatype or count");
     if (cfg->num_ports < 1 || cfg->num_ports > (uint32_t)kMaxPorts) return fail(BINE_ERR_ARG, "num_ports must be 1..6");
@@ -961,8 +966,10 @@ extern "C" int bine_plan_dump(const bine_vcomm_config_t* cfg, int32_t coll, int 
     Lowered low;
     if (exec == BINE_EXEC_FUSED)
         lower_groups(std::vector<std::vector<Group>>(1, cp.fused), tile_elems, (int)(16 / ds), 0, ds, &low, deep, split_lg(sh),
-                     nullptr, nullptr, two_phase);
-    else lower_groups(cp.step_groups, tile_elems, (int)(16 / ds), 0, ds, &low, deep, split_lg(sh), nullptr, nullptr, two_phase);
+                     nullptr, nullptr, two_phase & 3, (two_phase & 4) ? 2 : 1);
+    else
+        lower_groups(cp.step_groups, tile_elems, (int)(16 / ds), 0, ds, &low, deep, split_lg(sh), nullptr, nullptr,
+                     two_phase & 3, (two_phase & 4) ? 2 : 1);
     info->n_items = low.items.size();
     info->n_pool = low.pool.size();
     info->n_launches = low.launches.size();
@@ -1141,7 +1148,7 @@ int lower_list(bine_batch_t b, bine_batch_s::TypeList& l) {
         std::vector<uint8_t> bucket_vec(b->tabs.size());
         for (size_t i = 0; i < b->tabs.size(); ++i) bucket_vec[i] = (uint8_t)vec_ok(b->tabs[i]);
         lower_groups(l.groups, shallow_tile_elems(l.shape, ds), (int)(16 / ds), 0, ds, &all, deep_tile_elems(l.shape, ds),
-                     split_lg(l.shape), &l.group_bucket, &bucket_vec, l.shape.two_phase);
+                     split_lg(l.shape), &l.group_bucket, &bucket_vec, l.shape.two_phase, l.shape.double_tiles ? 2 : 1);
         int max_lg = max_lg_of(l.groups);
         bool tables_off16 = false;
         for (uint32_t gb : l.group_bucket) tables_off16 = tables_off16 || off16(b->tabs[gb]);
@@ -1328,13 +1335,14 @@ extern "C" int bine_batch_set_tuning(bine_batch_t b, int32_t vpt, int32_t thread
                                      int32_t flags) {
     if (!b) return fail(BINE_ERR_ARG, "NULL batch");
     if ((vpt != 1 && vpt != 2 && vpt != 4) || threads < 64 || threads > 512 || (threads & (threads - 1)) ||
-        persistent_wgs < 0 || flags < 0 || flags > 5)
+        persistent_wgs < 0 || flags < 0 || flags > 15 || ((flags >> 1) & 3) == 3)
         return fail(BINE_ERR_ARG, "bad launch shape");
     const LaunchShape sh = shape_from_words(vpt, threads, persistent_wgs, flags);
     b->shape = sh;
     drop_graph(b);
     for (auto& l : b->lists) {
-        const bool relower = sh.vpt != l.shape.vpt || sh.threads != l.shape.threads || sh.two_phase != l.shape.two_phase;
+        const bool relower = sh.vpt != l.shape.vpt || sh.threads != l.shape.threads || sh.two_phase != l.shape.two_phase ||
+                             sh.double_tiles != l.shape.double_tiles;
         l.shape = sh;
         if (relower) {
             int rc = lower_list(b, l);
@@ -1424,7 +1432,8 @@ extern "C" int bine_batch_autotune(bine_batch_t b, void* stream, int iters) {
     std::vector<std::vector<std::pair<float, LaunchShape>>> timed(nl);
     auto trial = [&](size_t i, const LaunchShape& c) -> int {
         auto& l = b->lists[i];
-        const bool relower = c.vpt != l.shape.vpt || c.threads != l.shape.threads || c.two_phase != l.shape.two_phase;
+        const bool relower = c.vpt != l.shape.vpt || c.threads != l.shape.threads || c.two_phase != l.shape.two_phase ||
+                             c.double_tiles != l.shape.double_tiles;
         l.shape = c;
         int rc = relower ? lower_list(b, l) : BINE_OK;
         if (rc) return rc;
@@ -1454,7 +1463,7 @@ extern "C" int bine_batch_autotune(bine_batch_t b, void* stream, int iters) {
     for (size_t i = 0; i < nl; ++i) {
         auto& l = b->lists[i];
         const bool relower = pick[i].vpt != l.shape.vpt || pick[i].threads != l.shape.threads ||
-                             pick[i].two_phase != l.shape.two_phase;
+                             pick[i].two_phase != l.shape.two_phase || pick[i].double_tiles != l.shape.double_tiles;
         l.shape = pick[i];
         if (relower) {
             int rc = lower_list(b, l);

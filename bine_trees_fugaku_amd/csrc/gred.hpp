@@ -56,13 +56,17 @@ struct LaunchShape {
     int two_phase = 0;    // reductions with several outputs: 0 one launch; 1 two launches (the reductions
                           // write their first output, then copies of it fill the others), copies in group
                           // order; 2 the same, copies tile by tile
+    int double_tiles = 0; // two phases: the reduction launch's items span two tiles each
 };
 
-// The fourth launch-shape word of the C interface: bit 0 streaming, bits 1-2 the two-phase mode.
+// The fourth launch-shape word of the C interface: bit 0 streaming, bits 1-2 the two-phase mode, bit 3 reduction
+// items of two tiles in a two-phase list.
 inline LaunchShape shape_from_words(int vpt, int threads, int persistent, int flags) {
-    return LaunchShape{vpt, threads, persistent, flags & 1, (flags >> 1) & 3};
+    return LaunchShape{vpt, threads, persistent, flags & 1, (flags >> 1) & 3, (flags >> 3) & 1};
 }
-inline int shape_flags(const LaunchShape& s) { return (s.nontemporal ? 1 : 0) | (s.two_phase << 1); }
+inline int shape_flags(const LaunchShape& s) {
+    return (s.nontemporal ? 1 : 0) | (s.two_phase << 1) | (s.double_tiles ? 8 : 0);
+}
 
 // Items whose tree has at least 2^(log2(waves) + 3) leaves are reduced with
 // their leaves split across the waves of the workgroup; their tile is one

@@ -1702,8 +1702,10 @@ bool compile_program(const Program& prog, bool want_fused, CompiledPlan* out, st
 
 void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shallow_tile, int vec, uint32_t bucket,
                   size_t dtsize, Lowered* out, int64_t deep_tile_elems, int deep_lg,
-                  const std::vector<uint32_t>* group_bucket, const std::vector<uint8_t>* bucket_vec, int two_phase) {
-    auto lower_launch = [&](const std::vector<Group>& gs, const std::vector<uint32_t>* gbk, bool copies_by_tile) {
+                  const std::vector<uint32_t>* group_bucket, const std::vector<uint8_t>* bucket_vec, int two_phase,
+                  int reduction_tiles) {
+    auto lower_launch = [&](const std::vector<Group>& gs, const std::vector<uint32_t>* gbk, bool copies_by_tile,
+                            int64_t tiles_per_item) {
         const uint32_t first = (uint32_t)out->items.size();
         // Open run of aligned single copies: consecutive groups whose leaf and
         // output advance by one fixed (rank, offset) step become one ITEM_RUN item.
@@ -1767,7 +1769,8 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
             }
             run_item = -1;
             run_len = 0;
-            const int64_t tile_elems = (g.lg >= deep_lg && deep_tile_elems > 0) ? deep_tile_elems : shallow_tile;
+            const int64_t tile_elems =
+                ((g.lg >= deep_lg && deep_tile_elems > 0) ? deep_tile_elems : shallow_tile) * tiles_per_item;
             const uint32_t leaf_off = (uint32_t)out->pool.size();
             out->pool.insert(out->pool.end(), g.leaves.begin(), g.leaves.end());
             const uint32_t out_off = (uint32_t)out->pool.size();
@@ -1883,7 +1886,7 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
         bool split = false;
         for (const Group& g : gs) split = split || (two_phase != 0 && g.lg > 0 && g.outs.size() > 1);
         if (!split) {
-            lower_launch(gs, gb, false);
+            lower_launch(gs, gb, false, 1);
             continue;
         }
         // Two phases: each reduction writes its first output only, and a second launch copies that output to
@@ -1907,8 +1910,10 @@ void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t shall
                 bc_b.push_back(b);
             }
         }
-        lower_launch(red, &red_b, false);
-        lower_launch(bc, &bc_b, two_phase == 2);
+        // reduction items of two tiles: half as many items, so a list of many tiles per block runs in fewer
+        // rounds of resident workgroups (its reads then meet fewer of the reductions' writes)
+        lower_launch(red, &red_b, false, reduction_tiles == 2 ? 2 : 1);
+        lower_launch(bc, &bc_b, two_phase == 2, 1);
     }
 }
 

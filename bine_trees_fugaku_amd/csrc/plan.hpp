@@ -121,11 +121,12 @@ struct Lowered {
 // bucket_vec: per buffer table, whether its addresses allow 16-byte vector access (no ITEM_VEC otherwise).
 // two_phase (1 or 2): a launch holding reductions with several outputs becomes two launches, the reductions
 // writing their first output, then copies of it to the other outputs (1: copies in group order, 2: tile by
-// tile across the groups, as the reductions).
+// tile across the groups, as the reductions).  reduction_tiles: tiles per item in the reduction launch of a
+// two-phase lowering (1 or 2).
 void lower_groups(const std::vector<std::vector<Group>>& launches, int64_t tile_elems, int vec_elems,
                   uint32_t bucket, size_t dtsize, Lowered* out, int64_t deep_tile_elems = 0, int deep_lg = 99,
                   const std::vector<uint32_t>* group_bucket = nullptr, const std::vector<uint8_t>* bucket_vec = nullptr,
-                  int two_phase = 0);
+                  int two_phase = 0, int reduction_tiles = 1);
 // Whether some launch holds a reduction with several outputs (the lists a two-phase lowering changes).
 bool has_multi_output_reduction(const std::vector<std::vector<Group>>& launches);
 

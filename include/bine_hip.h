@@ -228,7 +228,8 @@ int bine_plan_stats(bine_vcomm_t comm, int32_t coll, int count, bine_datatype_t 
  * and keep the fastest for later calls.  Runs the collective (iters + 1) times
  * per candidate (bine_tuning_candidates lists them; 28 on an MI355X), then,
  * where the schedule holds reductions with several outputs, at the three
- * fastest shapes in both two-phase modes (six more; see bine_batch_set_tuning).  The
+ * fastest shapes in both two-phase modes, with reduction items of one and two
+ * tiles (twelve more; see bine_batch_set_tuning).  The
  * work list in place before the call stays allocated until the vcomm is
  * destroyed, so graphs captured earlier keep running the old launch shape.
  * Must not run concurrently with other calls on the same vcomm. */
@@ -241,7 +242,8 @@ int bine_plan_autotune(bine_vcomm_t comm, int32_t coll, const void* sendbuf, voi
  * gather-reduce items); pool: info->n_pool records of 8 bytes; launches:
  * info->n_launches pairs (first item, item count).  in_place: the allreduce
  * schedule for sendbuf == recvbuf (SEND has no extent).  two_phase: 0, or the
- * two-phase mode 1 or 2 of bine_batch_set_tuning.  No HIP call is made. */
+ * two-phase mode 1 or 2 of bine_batch_set_tuning, plus 4 for reduction items of
+ * two tiles (the flags word shifted right by one).  No HIP call is made. */
 typedef struct {
     uint64_t n_items, n_pool, n_launches;
     int64_t extent[4];  /* per-rank elements of SEND, RECV, TMP, TMP2 */
@@ -281,7 +283,7 @@ int bine_batch_launch(bine_batch_t batch, void* stream);
 int bine_batch_launch_graph(bine_batch_t batch, void* stream);
 /* Time candidate (vector width, workgroup, persistence, streaming) shapes and keep the fastest, chosen
  * separately for each element type's work list; a list holding reductions with several outputs is also
- * timed at its three fastest shapes in both two-phase modes. */
+ * timed at its three fastest shapes in both two-phase modes, with reduction items of one and two tiles. */
 int bine_batch_autotune(bine_batch_t batch, void* stream, int iters);
 int bine_batch_stats(bine_batch_t batch, bine_plan_stats_t* out);
 /* Launch shape: 16-byte vectors per thread (1, 2, 4), threads per workgroup (64..512, power of two),
@@ -290,12 +292,13 @@ int bine_batch_stats(bine_batch_t batch, bine_plan_stats_t* out);
  * 0 one launch, the reductions writing every output; BINE_TUNE_TWO_PHASE (2) the reductions write their
  * first output, then a second launch copies it to the others, so the memory serves a read phase and a
  * write phase instead of both at once (same bits); BINE_TUNE_TWO_PHASE_BY_TILE (4) the same with the
- * copies issued tile by tile across the blocks instead of block by block.  get returns the last shape
+ * copies issued tile by tile across the blocks instead of block by block; BINE_TUNE_DOUBLE_TILES (8), with a
+ * two-phase mode, gives the reduction launch items of two tiles (half as many).  get returns the last shape
  * set, or after autotuning the shape of the list that takes longest, or before either the first list's
  * starting shape (4, 64, 0, streaming where reductions of 64 leaves or more over blocks of two whole
  * tiles or more hold 90 % of the list's bytes in at least 1024 tiles; 1, 256, 0, 0 otherwise); set
  * applies one shape to every list. */
-enum { BINE_TUNE_NONTEMPORAL = 1, BINE_TUNE_TWO_PHASE = 2, BINE_TUNE_TWO_PHASE_BY_TILE = 4 };
+enum { BINE_TUNE_NONTEMPORAL = 1, BINE_TUNE_TWO_PHASE = 2, BINE_TUNE_TWO_PHASE_BY_TILE = 4, BINE_TUNE_DOUBLE_TILES = 8 };
 int bine_batch_get_tuning(bine_batch_t batch, int32_t* vpt, int32_t* threads, int32_t* persistent_wgs,
                           int32_t* flags);
 int bine_batch_set_tuning(bine_batch_t batch, int32_t vpt, int32_t threads, int32_t persistent_wgs,

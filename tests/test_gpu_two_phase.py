@@ -14,10 +14,12 @@ from tests.test_gpu_batch import bucket_sizes, mixed_items, randint, randn, same
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("double", [0, 1])
 @pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("coll", ["allreduce_f32", "allreduce_f64", "allreduce_i8"])
-def test_every_candidate_shape_two_phase(gpu, coll, mode):
-    """Odd counts on 4x4x4: blocks start 4, 8 or 12 bytes past 16-byte boundaries and end in ragged tails."""
+def test_every_candidate_shape_two_phase(gpu, coll, mode, double):
+    """Odd counts on 4x4x4: blocks start 4, 8 or 12 bytes past 16-byte boundaries and end in ragged tails.
+    double: reduction items of two tiles."""
     from bine_trees_fugaku_amd.collectives import tuning_candidates
     P, dims = 64, [4, 4, 4]
     dt, c = {"allreduce_f32": (torch.float32, 64 * 37 + 5), "allreduce_f64": (torch.float64, 64 * 29 + 3),
@@ -29,12 +31,12 @@ def test_every_candidate_shape_two_phase(gpu, coll, mode):
         b = Batch(comm, [{"coll": "allreduce", "sendbuf": send, "recvbuf": recv, "count": c}])
         for vpt, thr, pers, nt in tuning_candidates():
             recv.zero_()
-            b.set_tuning(int(vpt), int(thr), int(pers), int(nt), two_phase=mode)
-            assert b.tuning()["two_phase"] == mode
+            b.set_tuning(int(vpt), int(thr), int(pers), int(nt), two_phase=mode, double_tiles=double)
+            assert b.tuning()["two_phase"] == mode and b.tuning()["double_tiles"] == double
             b.launch()
             torch.cuda.synchronize()
             got = recv.cpu().numpy()
-            assert got.tobytes() == want.reshape(got.shape).tobytes(), (coll, mode, vpt, thr, pers, nt)
+            assert got.tobytes() == want.reshape(got.shape).tobytes(), (coll, mode, double, vpt, thr, pers, nt)
         assert b.stats()["launches"] == 2
         b.close()
 
@@ -52,14 +54,14 @@ def test_two_phase_512_ranks_launch_and_graph(gpu):
         ref = recv.clone()
         col = O.allreduce(send[:, :P].cpu().numpy().copy(), dims)  # a 512-element allreduce, one element per block
         for mode in (1, 2):
-            for vpt, thr, pers, nt in [(4, 64, 0, 1), (1, 512, 0, 0), (2, 128, 1024, 1), (4, 256, 0, 1)]:
-                b.set_tuning(vpt, thr, pers, nt, two_phase=mode)
+            for vpt, thr, pers, nt, dbl in [(4, 64, 0, 1, 0), (1, 512, 0, 0, 1), (2, 128, 1024, 1, 0), (4, 256, 0, 1, 1)]:
+                b.set_tuning(vpt, thr, pers, nt, two_phase=mode, double_tiles=dbl)
                 for k in range(3):  # graph replays with eager work between them
                     recv.fill_(float("nan"))
                     torch.cuda.synchronize()
                     b.launch_graph() if k else b.launch()
                     torch.cuda.synchronize()
-                    assert same_bits(recv, ref), (mode, vpt, thr, pers, nt, k)
+                    assert same_bits(recv, ref), (mode, vpt, thr, pers, nt, dbl, k)
         b.close()
     # a two-phase list of one element per block against the oracle
     with VComm(dims, exec="FUSED") as comm:
@@ -87,8 +89,8 @@ def test_two_phase_uneven_buckets(gpu, mode):
         refs = [r.clone() for r in recvs]
         for i in (0, 5, 24, 25, 27):
             assert recvs[i].cpu().numpy().tobytes() == O.allreduce(sends[i].cpu().numpy(), dims).tobytes(), i
-        for shape in [(4, 64, 0, 1), (2, 256, 0, 1), (1, 512, 2048, 0)]:
-            b.set_tuning(*shape, two_phase=mode)
+        for shape, dbl in [((4, 64, 0, 1), 0), ((2, 256, 0, 1), 1), ((1, 512, 2048, 0), 0), ((4, 64, 0, 1), 1)]:
+            b.set_tuning(*shape, two_phase=mode, double_tiles=dbl)
             for k in range(2):
                 for r in recvs:
                     r.fill_(float("nan"))
@@ -96,7 +98,7 @@ def test_two_phase_uneven_buckets(gpu, mode):
                 b.launch_graph() if k else b.launch()
                 torch.cuda.synchronize()
                 for i, (r, ref) in enumerate(zip(recvs, refs)):
-                    assert same_bits(r, ref), (mode, shape, k, i, sizes[i])
+                    assert same_bits(r, ref), (mode, shape, dbl, k, i, sizes[i])
         b.close()
 
 

@@ -361,36 +361,40 @@ def test_family_default_runs_its_stand_in(coll, dims, standin):
 
 @pytest.mark.parametrize("dims,algo,ports", [([8], "B", 1), ([4, 4], "B", 1), ([4, 4, 4], "B_CONT", 1), ([2, 8, 2], "B", 1),
                                              ([4, 4], "B", 2), ([2, 2, 2], "B_COALESCE", 3), ([8, 8, 8], "B", 1)])
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 5, 6])
 @pytest.mark.parametrize("dtype", ["float32", "float64", "int32"])
 def test_two_phase_allreduce(dims, algo, ports, mode, dtype):
     """Two-phase lowering of the fused allreduce (the schedules that fuse: powers of two; L and non-powers of two
-    run per step): the first launch holds the reductions, each writing one output;
-    the second holds copies of that output to the block's other P - 1 outputs (group by group in mode 1, tile by
-    tile in mode 2).  Every input is read once, every output written once, and each block's sum once more read
-    back for the copies.  The emulated result equals the oracle bit for bit."""
+    run per step): the first launch holds the reductions, each writing one output; the second holds copies of
+    that output to the block's other P - 1 outputs (group by group in mode 1, tile by tile in mode 2; adding 4
+    gives the reductions items of two tiles).  Every input is read once, every output written once, and each
+    block's sum once more read back for the copies.  The emulated result equals the oracle bit for bit."""
     P = int(np.prod(dims))
-    count = 5 * P * ports + 3
+    count = (40 if P <= 64 else 5) * P * ports + 3
+    tile = 16
     x = data((P, count), dtype, P + mode)
     kw = dict(allreduce_algo=algo, num_ports=ports)
-    meta, items, pool, launches = plan_dump(dims, "allreduce", count, dtype, exec="FUSED", tile_elems=16,
+    meta, items, pool, launches = plan_dump(dims, "allreduce", count, dtype, exec="FUSED", tile_elems=tile,
                                             two_phase=mode, **kw)
     assert len(launches) == 2
     (f1, n1), (f2, n2) = launches
     first, second = items[f1:f1 + n1], items[f2:f2 + n2]
     red = first[first["lg"] > 0]
     assert len(red) > 0 and (red["nouts"] == 1).all()
-    assert (second["lg"] == 0).all() and (second["nouts"] == P - 1).all()
+    assert (red["n"] <= (2 if mode & 4 else 1) * tile).all()
+    if mode & 4 and P <= 64:
+        assert (red["n"] > tile).any()  # blocks of 2.5 tiles: some items span two
+    assert (second["lg"] == 0).all() and (second["nouts"] == P - 1).all() and (second["n"] <= tile).all()
     reads = sum(int(it["n"]) << int(it["lg"]) for it in items)
     writes = sum(int(it["n"]) * int(it["nouts"]) for it in items)
     assert reads == (P + 1) * count and writes == P * count
-    if mode == 1:  # group by group: one block's copies are consecutive
+    if mode & 3 == 1:  # group by group: one block's copies are consecutive
         assert (np.diff(second["leaf_off"].astype(np.int64)) >= 0).all()
     else:  # the same copies in another order
-        m1 = plan_dump(dims, "allreduce", count, dtype, exec="FUSED", tile_elems=16, two_phase=1, **kw)
+        m1 = plan_dump(dims, "allreduce", count, dtype, exec="FUSED", tile_elems=tile, two_phase=mode - 1, **kw)
         other = m1[1][m1[3][1][0]:]
         assert sorted(r.tobytes() for r in second) == sorted(r.tobytes() for r in other)
-    _, bufs = run_plan(dims, "allreduce", count, dtype, "FUSED", send=x, tile=16, two_phase=mode, **kw)
+    _, bufs = run_plan(dims, "allreduce", count, dtype, "FUSED", send=x, tile=tile, two_phase=mode, **kw)
     assert same_bits(bufs[E.BUF_RECV], O.allreduce(x, dims, algo=algo, ports=ports))
 
 

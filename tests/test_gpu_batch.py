@@ -317,3 +317,28 @@ def test_every_candidate_shape_gives_the_same_bits(gpu, coll):
             got = recv.cpu().numpy()
             assert got.tobytes() == want.reshape(got.shape).tobytes(), (coll, vpt, thr, pers, nt)
         b.close()
+
+
+def test_starting_shape_follows_block_length(gpu):
+    """Before any tuning a list of long blocks in at least 1024 tiles starts at four vectors per thread, one-wave
+    workgroups and streaming (4, 64, 0, 1); a list of one-tile blocks starts at the general shape (1, 256, 0, 0).
+    Both write the bits the other shape writes."""
+    P, dims = 64, [4, 4, 4]
+    for count, want in [(16 * 1024 * 64, (4, 64, 0, 1)), (1024 * 64, (1, 256, 0, 0))]:
+        send = randn((P, count), torch.float32, 31)
+        recv = torch.empty_like(send)
+        with VComm(dims, exec="FUSED") as comm:
+            b = Batch(comm, [{"coll": "allreduce", "sendbuf": send, "recvbuf": recv, "count": count}])
+            t = b.tuning()
+            assert (t["vpt"], t["threads"], t["persistent_wgs"], t["nontemporal"]) == want, (count, t)
+            assert t["two_phase"] == 0
+            b.launch()
+            torch.cuda.synchronize()
+            first = recv.clone()
+            other = (1, 256, 0, 0) if want != (1, 256, 0, 0) else (4, 64, 0, 1)
+            b.set_tuning(*other)
+            recv.zero_()
+            b.launch()
+            torch.cuda.synchronize()
+            assert same_bits(recv, first), count
+            b.close()
